@@ -1,0 +1,341 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X BLIS level-3 hot path.
+
+Default (N = 1): the headline workload is native DGEMM, column-major,
+no-transpose, m = n = k = 8192, A, B, C ~ U[-1,1] (seeded, generated on the
+device), alpha = 1, beta = 0.5, called through the C ABI (bli_dgemm) on
+device-resident operands. A step is one such call (synchronous, BLIS
+semantics). The JSON line reports:
+  value       whole-job TFLOP/s over the K timed steps (2mnk per step);
+  roofline    the GEMM kernel's TFLOP/s from HIP events on the stream the
+              kernel runs on, against the FP64 MFMA peak (78.6 TFLOP/s);
+  cpu_baseline the oracle's five-loop DGEMM timed on the host cores on a
+              bounded 2048^3 sample of the same workload;
+  extras      the other configured shapes (SGEMM, skinny k = 256, DTRSM,
+              DGEMV, DAXPYV, DDOTV, batched uneven GEMM) at a few reps each.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(name: str):
+    """HBM bytes per launch recorded by a rocprofv3 --pmc summary under
+    profiles/ (line 'traffic_bytes_per_launch: <int>'), else None."""
+    p = ROOT / "profiles" / name
+    if not p.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
+    return float(m.group(1)) if m else None
+
+
+class Bench:
+    def __init__(self):
+        import torch
+        import blis_amd
+        self.torch = torch
+        self.L = blis_amd.lib()
+        self.stream = torch.cuda.Stream()
+        # Run the library on a stream we can put events on.
+        self.L.bli_amd_set_stream(C.c_void_p(self.stream.cuda_stream), 1)
+        self.gen = torch.Generator(device="cuda")
+        self.gen.manual_seed(1234)
+
+    def uniform(self, n, dtype):
+        t = self.torch.empty(n, dtype=dtype, device="cuda")
+        t.uniform_(-1.0, 1.0, generator=self.gen)
+        return t
+
+    def kernel_ms(self, fn, reps):
+        """Average device time per call: calls enqueued back to back on the
+        library stream (no host sync in between), bracketed by HIP events."""
+        torch = self.torch
+        self.L.bli_amd_set_stream(C.c_void_p(self.stream.cuda_stream), 0)
+        try:
+            # torch work inside fn goes to the same stream as the library's.
+            with torch.cuda.stream(self.stream):
+                fn()
+                torch.cuda.synchronize()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(self.stream)
+                for _ in range(reps):
+                    fn()
+                e1.record(self.stream)
+            e1.synchronize()
+            return e0.elapsed_time(e1) / reps
+        finally:
+            self.L.bli_amd_set_stream(C.c_void_p(self.stream.cuda_stream), 1)
+
+    # ---- workloads ----
+
+    def gemm_fn(self, ch, ta, tb, m, n, k, alpha=1.0, beta=0.5):
+        torch = self.torch
+        dt = torch.float64 if ch == "d" else torch.float32
+        T = C.c_double if ch == "d" else C.c_float
+        am, an = (k, m) if ta else (m, k)
+        bm, bn = (n, k) if tb else (k, n)
+        a, b, c = self.uniform(am * an, dt), self.uniform(bm * bn, dt), self.uniform(m * n, dt)
+        al, be = T(alpha), T(beta)
+        f = getattr(self.L, f"bli_{ch}gemm")
+        args = (8 if ta else 0, 8 if tb else 0, m, n, k, C.byref(al), a.data_ptr(), 1, am,
+                b.data_ptr(), 1, bm, C.byref(be), c.data_ptr(), 1, m)
+        keep = (a, b, c, al, be)
+
+        def fn():
+            f(*args)
+        fn.keep = keep
+        return fn
+
+
+def cpu_baseline():
+    """The oracle's multithreaded five-loop DGEMM on the host cores, on a
+    2048^3 sample of the headline workload (same distribution, alpha = 1,
+    beta = 0.5)."""
+    import numpy as np
+    from oracle import pyoracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    s = 2048
+    rng = np.random.default_rng(7)
+    a = np.asfortranarray(rng.uniform(-1, 1, (s, s)))
+    b = np.asfortranarray(rng.uniform(-1, 1, (s, s)))
+    c = np.asfortranarray(rng.uniform(-1, 1, (s, s)))
+    pyoracle.gemm("n", "n", 1.0, a[:256, :256], b[:256, :256], 0.5, c[:256, :256].copy(order="F"),
+                  nthreads=threads)
+    best = float("inf")
+    for _ in range(2):
+        t0 = time.perf_counter()
+        pyoracle.gemm("n", "n", 1.0, a, b, 0.5, c, nthreads=threads)
+        best = min(best, time.perf_counter() - t0)
+    return {"value": round(2.0 * s ** 3 / best / 1e12, 6), "unit": "TFLOP/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle five-loop DGEMM (MR 6, NR 8, MC 72, KC 256, NC 4080; OpenMP over "
+                      f"the ic loop) on a {s}^3 NN sample, best of 2, {threads} threads"}
+
+
+def extras(bench: Bench):
+    torch = bench.torch
+    out = {}
+    L = bench.L
+
+    def rec(name, ms, work, unit, peak, kind):
+        val = work / (ms * 1e-3)
+        if unit == "TFLOP/s":
+            val /= 1e12
+        else:
+            val /= 1e9
+        out[name] = {"ms": round(ms, 4), unit: round(val, 3), "frac": round(val / peak, 4),
+                     "bound": kind}
+        log(f"  {name:34s} {ms:9.4f} ms  {val:9.3f} {unit}  ({100 * val / peak:5.1f}% of {peak})")
+
+    # Native level-3 at scale (config 2).
+    for ch, key in (("d", "f64"), ("s", "f32")):
+        for (ta, tb) in ((0, 0), (1, 0), (0, 1), (1, 1)):
+            for (m, n, k) in ((8192, 8192, 8192), (8192, 8192, 256)):
+                fn = bench.gemm_fn(ch, ta, tb, m, n, k)
+                reps = 5 if k == 8192 else 20
+                ms = bench.kernel_ms(fn, reps)
+                tag = f"{ch}gemm_{'nt'[ta]}{'nt'[tb]}_{m}x{n}x{k}"
+                rec(tag, ms, 2.0 * m * n * k, "TFLOP/s", PEAK_TFLOPS[key], "mfma")
+                del fn
+                torch.cuda.empty_cache()
+
+    # DTRSM left/lower/no-trans/non-unit, m = 4096, n = 1024.
+    m, n = 4096, 1024
+    # Row-major upper triangle == column-major lower triangle.
+    a = torch.triu(bench.uniform(m * m, torch.float64).view(m, m)).contiguous()
+    a.diagonal().add_(4096.0)
+    b0 = bench.uniform(m * n, torch.float64)
+    b = b0.clone()
+    al = C.c_double(1.0)
+
+    def trsm():
+        b.copy_(b0)
+        L.bli_dtrsm(0, 0xC0, 0, 0, m, n, C.byref(al), a.data_ptr(), 1, m, b.data_ptr(), 1, m)
+    ms_copy = bench.kernel_ms(lambda: b.copy_(b0), 10)
+    ms = bench.kernel_ms(trsm, 10) - ms_copy
+    rec("dtrsm_llnn_4096x1024", ms, float(m) * m * n, "TFLOP/s", PEAK_TFLOPS["f64"], "mfma")
+    del a, b, b0
+    torch.cuda.empty_cache()
+
+    # DGEMV 16384^2 (N and T), column-major.
+    m = n = 16384
+    a = bench.uniform(m * n, torch.float64)
+    x = bench.uniform(n, torch.float64)
+    y = bench.uniform(m, torch.float64)
+    one, half = C.c_double(1.0), C.c_double(0.5)
+    for t, name in ((0, "dgemv_n_16384"), (8, "dgemv_t_16384")):
+        fn = lambda t=t: L.bli_dgemv(t, 0, m, n, C.byref(one), a.data_ptr(), 1, m, x.data_ptr(), 1,
+                                     C.byref(half), y.data_ptr(), 1)
+        ms = bench.kernel_ms(fn, 20)
+        rec(name, ms, 8.0 * (m * n + n + 2 * m), "GB/s", PEAK_HBM_GBS, "hbm")
+    del a, x, y
+    torch.cuda.empty_cache()
+
+    # DAXPYV / DDOTV n = 2^28.
+    n = 1 << 28
+    x = bench.uniform(n, torch.float64)
+    y = bench.uniform(n, torch.float64)
+    al = C.c_double(0.5)
+    ms = bench.kernel_ms(lambda: L.bli_daxpyv(0, n, C.byref(al), x.data_ptr(), 1, y.data_ptr(), 1), 20)
+    rec("daxpyv_2^28", ms, 24.0 * n, "GB/s", PEAK_HBM_GBS, "hbm")
+    rho = torch.zeros(1, dtype=torch.float64, device="cuda")
+    ms = bench.kernel_ms(lambda: L.bli_ddotv(0, 0, n, x.data_ptr(), 1, y.data_ptr(), 1,
+                                             C.c_void_p(rho.data_ptr())), 20)
+    rec("ddotv_2^28", ms, 16.0 * n, "GB/s", PEAK_HBM_GBS, "hbm")
+    del x, y
+    torch.cuda.empty_cache()
+
+    # Batched uneven GEMM (config 4): 10,000 problems, m, n, k log-uniform in [8, 512].
+    for ch, key in (("d", "f64"), ("s", "f32")):
+        out.update(batch_bench(bench, ch, key, log))
+    return out
+
+
+def batch_bench(bench, ch, key, log, count=10000, seed=42):
+    import numpy as np
+    torch = bench.torch
+    L = bench.L
+    rng = np.random.default_rng(seed)
+    dims = np.rint(np.exp(rng.uniform(np.log(8), np.log(512), (count, 3)))).astype(np.int64)
+    ta = rng.integers(0, 2, count)
+    tb = rng.integers(0, 2, count)
+    alpha = rng.uniform(-1, 1, count)
+    beta = rng.uniform(-1, 1, count)
+    es = 8 if ch == "d" else 4
+    dt = torch.float64 if ch == "d" else torch.float32
+    pad = lambda v: (v + 7) // 8 * 8
+    lda = np.array([pad(k if t else m) for (m, n, k), t in zip(dims, ta)], dtype=np.int64)
+    ldb = np.array([pad(n if t else k) for (m, n, k), t in zip(dims, tb)], dtype=np.int64)
+    ldc = np.array([pad(m) for (m, n, k) in dims], dtype=np.int64)
+    acols = np.array([(m if t else k) for (m, n, k), t in zip(dims, ta)])
+    bcols = np.array([(k if t else n) for (m, n, k), t in zip(dims, tb)])
+    sa, sb, sc = lda * acols, ldb * bcols, ldc * dims[:, 1]
+    # One arena per operand kind; each matrix starts on a 256-byte boundary.
+    al = 256 // es
+    offs = lambda sizes: np.concatenate([[0], np.cumsum((sizes + al - 1) // al * al)])
+    oa, ob, oc = offs(sa), offs(sb), offs(sc)
+    A = bench.uniform(int(oa[-1]), dt)
+    B = bench.uniform(int(ob[-1]), dt)
+    Cm = bench.uniform(int(oc[-1]), dt)
+    T = C.c_double if ch == "d" else C.c_float
+    I = C.c_int64 * count
+    Pp = C.c_void_p * count
+    args = [(C.c_int * count)(*(ta * 8).tolist()), (C.c_int * count)(*(tb * 8).tolist()),
+            I(*dims[:, 0].tolist()), I(*dims[:, 1].tolist()), I(*dims[:, 2].tolist()),
+            (T * count)(*alpha.tolist()),
+            Pp(*[A.data_ptr() + int(o) * es for o in oa[:-1]]), I(*([1] * count)), I(*lda.tolist()),
+            Pp(*[B.data_ptr() + int(o) * es for o in ob[:-1]]), I(*([1] * count)), I(*ldb.tolist()),
+            (T * count)(*beta.tolist()),
+            Pp(*[Cm.data_ptr() + int(o) * es for o in oc[:-1]]), I(*([1] * count)), I(*ldc.tolist())]
+    gs = I(*([1] * count))
+    cargs = [C.cast(x, C.c_void_p) for x in args] + [count, C.cast(gs, C.c_void_p)]
+    plan = getattr(L, f"bli_amd_{ch}gemm_batch_plan_create")(*cargs)
+    ms = bench.kernel_ms(lambda: L.bli_amd_gemm_batch_plan_execute(plan, 1), 20)
+    L.bli_amd_gemm_batch_plan_destroy(plan)
+    m, n, k = dims[:, 0].astype(float), dims[:, 1].astype(float), dims[:, 2].astype(float)
+    flops = 2 * m * n * k
+    byts = es * (m * k + k * n + 2 * m * n)
+    peak = PEAK_TFLOPS[key] * 1e12
+    roof_ms = float(np.sum(np.maximum(flops / peak, byts / (PEAK_HBM_GBS * 1e9)))) * 1e3
+    tf = flops.sum() / (ms * 1e-3) / 1e12
+    res = {f"{ch}gemm_batch_10000": {"ms": round(ms, 4), "TFLOP/s": round(tf, 3),
+                                     "roofline_ms": round(roof_ms, 4),
+                                     "frac": round(roof_ms / ms, 4),
+                                     "bound": "per-problem max(mfma, hbm)"}}
+    log(f"  {ch}gemm_batch_10000{'':17s} {ms:9.4f} ms  {tf:9.3f} TFLOP/s  "
+        f"({100 * roof_ms / ms:5.1f}% of per-problem roofline {roof_ms:.3f} ms)")
+    del A, B, Cm
+    torch.cuda.empty_cache()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(the hot path is single-GPU by design)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    bench = Bench()
+    m = n = k = 8192
+    log(f"headline: DGEMM NN {m}x{n}x{k}, alpha=1, beta=0.5, warmup {args.warmup}, steps {args.steps}")
+    step = bench.gemm_fn("d", 0, 0, m, n, k)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    flops = 2.0 * m * n * k
+    value = flops / (ms_per_step * 1e-3) / 1e12
+    kms = bench.kernel_ms(step, max(5, args.steps // 2))
+    achieved = flops / (kms * 1e-3) / 1e12
+    log(f"  dgemm 8192^3: {ms_per_step:.3f} ms/step ({value:.2f} TFLOP/s), kernel {kms:.3f} ms "
+        f"({achieved:.2f} TFLOP/s, {100 * achieved / PEAK_TFLOPS['f64']:.1f}% of FP64 MFMA peak)")
+    del step
+    torch.cuda.empty_cache()
+
+    extra = {} if args.no_extras else extras(bench)
+    cpu = None if args.no_cpu else cpu_baseline()
+    if cpu:
+        log(f"  cpu baseline: {cpu['value']} TFLOP/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": "DGEMM/SGEMM TFLOP/s (% MFMA peak): native vs split-precision at matched error",
+        "value": round(value, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: U[-1,1] seeded on device",
+        "config": {"workload": "native DGEMM NN column-major m=n=k=8192, alpha=1, beta=0.5, "
+                               "device-resident operands, bli_dgemm",
+                   "m": m, "n": n, "k": k, "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS["f64"],
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS["f64"], 4),
+                     "traffic": traffic_from_profiles("dgemm_8192_nn_traffic.txt")},
+        "cpu_baseline": cpu,
+        "extras": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,270 @@
+// gemm.hip — gfx950 GEMM launch planner and kernels.
+//
+// Host side of the BLIS five-loop GEMM (frame/3/gemm/bli_gemm_cntl.c:75-565,
+// sandbox/gemmlike/bls_gemm_bp_var1.c:37-318): it normalises operand
+// storage, picks a tile configuration (the analogue of BLIS's MC/NC/KC/MR/NR
+// blocksizes, config/zen3/bli_cntx_init_zen3.c:204-220) and launches one
+// workgroup per output tile. The device side lives in gemm_tile.hpp.
+//
+// Storage normalisation follows bli_gemm_cntl_init's operation transpose
+// (bli_gemm_cntl.c:104-161): when C is row-stored the problem is recast as
+// C^T := B^T A^T so that the kernel always writes column-stored C. Operands
+// with neither unit row nor unit column stride are first packed into a
+// contiguous column-major copy, as packm does for general-stride inputs
+// (ref_kernels/1m/bli_packm_cxk_ref.c:78-147).
+#include "internal.hpp"
+#include "gemm_tile.hpp"
+#include "gemm_tiles.hpp"
+
+#include <algorithm>
+
+namespace blis_amd {
+
+using namespace dev;
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+__device__ inline void tile_coords(int bid, int nblocks, const GridInfo& g, int& tm, int& tn) {
+    if (g.xcd_remap) {
+        // Workgroups are dispatched round-robin over the 8 XCDs. Give each
+        // XCD a contiguous range of logical tile ids so that tiles sharing
+        // rows of A / columns of B hit the same L2.
+        const int X = 8;
+        const int q = nblocks / X, r = nblocks % X;
+        const int x = bid % X, l = bid / X;
+        bid = x * q + (x < r ? x : r) + l;
+    }
+    const int per_group = g.group_m * g.grid_n;
+    const int group = bid / per_group;
+    const int first_m = group * g.group_m;
+    const int gsz = min(g.grid_m - first_m, g.group_m);
+    const int in = bid - group * per_group;
+    tm = first_m + in % gsz;
+    tn = in / gsz;
+}
+
+template <typename T, class TL, bool A_K, bool B_K>
+__global__ __launch_bounds__(TL::NT) void gemm_kernel(TileProblem<T> p, GridInfo g) {
+    __shared__ T smem[2 * TL::STAGE];
+    int tm, tn;
+    tile_coords(blockIdx.x, gridDim.x, g, tm, tn);
+    gemm_tile<T, TL, A_K, B_K>(p, tm, tn, smem);
+}
+
+// C := beta*C for alpha == 0 or k == 0 (bli_l3_return_early_if_trivial,
+// frame/3/bli_l3_util.c:41-65). beta == 0 stores zeros without reading C.
+template <typename T>
+__global__ void scalm_kernel(T* c, int64_t m, int64_t n, int64_t rs, int64_t cs, T beta) {
+    const int64_t total = m * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = t % m, j = t / m;
+        T* dst = c + i * rs + j * cs;
+        *dst = (beta == T(0)) ? T(0) : beta * *dst;
+    }
+}
+
+// dst (column-major m x n, ld = m) := src with arbitrary strides.
+template <typename T>
+__global__ void pack_kernel(const T* src, int64_t m, int64_t n, int64_t rs, int64_t cs, T* dst) {
+    const int64_t total = m * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = t % m, j = t / m;
+        dst[t] = src[i * rs + j * cs];
+    }
+}
+
+static int grid_for(int64_t total) {
+    int64_t g = (total + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > 8192) g = 8192;
+    return (int)g;
+}
+
+void scalm_run(Dt dt, int64_t m, int64_t n, double beta, View c, hipStream_t st) {
+    if (m <= 0 || n <= 0) return;
+    const int g = grid_for(m * n);
+    if (dt == Dt::F64)
+        hipLaunchKernelGGL(scalm_kernel<double>, dim3(g), dim3(256), 0, st,
+                           (double*)c.ptr, m, n, c.rs, c.cs, beta);
+    else
+        hipLaunchKernelGGL(scalm_kernel<float>, dim3(g), dim3(256), 0, st,
+                           (float*)c.ptr, m, n, c.rs, c.cs, (float)beta);
+    HIP_CHECK(hipGetLastError());
+}
+
+static void pack_run(Dt dt, const void* src, int64_t m, int64_t n, int64_t rs, int64_t cs,
+                     void* dst, hipStream_t st) {
+    const int g = grid_for(m * n);
+    if (dt == Dt::F64)
+        hipLaunchKernelGGL(pack_kernel<double>, dim3(g), dim3(256), 0, st,
+                           (const double*)src, m, n, rs, cs, (double*)dst);
+    else
+        hipLaunchKernelGGL(pack_kernel<float>, dim3(g), dim3(256), 0, st,
+                           (const float*)src, m, n, rs, cs, (float*)dst);
+    HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Launch plumbing
+// ---------------------------------------------------------------------------
+
+template <typename T, class TL>
+static void launch_tl(const TileProblem<T>& p, bool a_k, bool b_k, const GridInfo& g,
+                      int nblocks, hipStream_t st) {
+    dim3 grid(nblocks), block(TL::NT);
+    if (!a_k && !b_k)
+        hipLaunchKernelGGL((gemm_kernel<T, TL, false, false>), grid, block, 0, st, p, g);
+    else if (!a_k && b_k)
+        hipLaunchKernelGGL((gemm_kernel<T, TL, false, true>), grid, block, 0, st, p, g);
+    else if (a_k && !b_k)
+        hipLaunchKernelGGL((gemm_kernel<T, TL, true, false>), grid, block, 0, st, p, g);
+    else
+        hipLaunchKernelGGL((gemm_kernel<T, TL, true, true>), grid, block, 0, st, p, g);
+    HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+static void launch_tile(int tile, const TileProblem<T>& p, bool a_k, bool b_k,
+                        GridInfo g, hipStream_t st) {
+    const TileShape& ts = tile_shape(dt_of<T>(), tile);
+    g.grid_m = (int)((p.m + ts.bm - 1) / ts.bm);
+    g.grid_n = (int)((p.n + ts.bn - 1) / ts.bn);
+    if (g.group_m < 1) g.group_m = 1;
+    const int nblocks = g.grid_m * g.grid_n;
+    if constexpr (std::is_same<T, double>::value) {
+        switch (tile) {
+#define BLIS_AMD_CASE(idx, BM, BN, BK, WM, WN) \
+        case idx: launch_tl<double, TileCfg<double, BM, BN, BK, WM, WN>>(p, a_k, b_k, g, nblocks, st); return;
+        BLIS_AMD_TILES_F64(BLIS_AMD_CASE)
+#undef BLIS_AMD_CASE
+        }
+    } else {
+        switch (tile) {
+#define BLIS_AMD_CASE(idx, BM, BN, BK, WM, WN) \
+        case idx: launch_tl<float, TileCfg<float, BM, BN, BK, WM, WN>>(p, a_k, b_k, g, nblocks, st); return;
+        BLIS_AMD_TILES_F32(BLIS_AMD_CASE)
+#undef BLIS_AMD_CASE
+        }
+    }
+    abort_with(BLIS_INVALID_UKR_ID, "gemm: tile index out of range");
+}
+
+static inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+// Normalised GEMM ready to launch: A mode, B mode, leading dimensions.
+struct Plan {
+    int64_t m, n, k;
+    const void* a; int64_t lda; bool a_k;
+    const void* b; int64_t ldb; bool b_k;
+    void* c; int64_t c_si, c_sj;
+};
+
+static Plan normalise(const GemmDesc& d0, hipStream_t st) {
+    GemmDesc d = d0;
+    // Row-stored C: compute C^T = B^T A^T instead.
+    if (d.c.rs != 1 && d.c.cs == 1) {
+        GemmDesc t = d;
+        t.m = d.n; t.n = d.m;
+        t.a.ptr = d.b.ptr; t.a.rs = d.b.cs; t.a.cs = d.b.rs;
+        t.b.ptr = d.a.ptr; t.b.rs = d.a.cs; t.b.cs = d.a.rs;
+        t.c.rs = d.c.cs; t.c.cs = d.c.rs;
+        d = t;
+    }
+    Plan p;
+    p.m = d.m; p.n = d.n; p.k = d.k;
+    p.c = d.c.ptr; p.c_si = d.c.rs; p.c_sj = d.c.cs;
+    const size_t es = dt_size(d.dt);
+    if (d.a.rs == 1) { p.a = d.a.ptr; p.lda = d.a.cs; p.a_k = false; }
+    else if (d.a.cs == 1) { p.a = d.a.ptr; p.lda = d.a.rs; p.a_k = true; }
+    else {
+        void* w = workspace((size_t)d.m * d.k * es, 0);
+        pack_run(d.dt, d.a.ptr, d.m, d.k, d.a.rs, d.a.cs, w, st);
+        p.a = w; p.lda = d.m; p.a_k = false;
+    }
+    if (d.b.rs == 1) { p.b = d.b.ptr; p.ldb = d.b.cs; p.b_k = true; }
+    else if (d.b.cs == 1) { p.b = d.b.ptr; p.ldb = d.b.rs; p.b_k = false; }
+    else {
+        void* w = workspace((size_t)d.k * d.n * es, 1);
+        pack_run(d.dt, d.b.ptr, d.k, d.n, d.b.rs, d.b.cs, w, st);
+        p.b = w; p.ldb = d.k; p.b_k = true;
+    }
+    return p;
+}
+
+template <typename T>
+static TileProblem<T> make_problem(const Plan& pl, double alpha, double beta) {
+    constexpr int V = Mma<T>::V;
+    TileProblem<T> p;
+    p.a = (const T*)pl.a; p.b = (const T*)pl.b; p.c = (T*)pl.c;
+    p.m = pl.m; p.n = pl.n; p.k = pl.k;
+    p.lda = pl.lda; p.ldb = pl.ldb;
+    p.c_si = pl.c_si; p.c_sj = pl.c_sj;
+    p.alpha = (T)alpha; p.beta = (T)beta;
+    p.vec_a = aligned16(pl.a) && (pl.lda % V == 0);
+    p.vec_b = aligned16(pl.b) && (pl.ldb % V == 0);
+    p.vec_c = aligned16(pl.c) && pl.c_si == 1 && (pl.c_sj % V == 0);
+    return p;
+}
+
+int gemm_default_tile(Dt dt, int64_t m, int64_t n, int64_t k) {
+    // Largest tile that still yields at least ~2 workgroups per CU; smaller
+    // tiles otherwise, so that small problems spread over more CUs.
+    (void)k;
+    const int ntiles = tile_count(dt);
+    int best = ntiles - 1;
+    for (int t = 0; t < ntiles; ++t) {
+        const TileShape& ts = tile_shape(dt, t);
+        if (!ts.heuristic) continue;
+        const int64_t nb = ((m + ts.bm - 1) / ts.bm) * ((n + ts.bn - 1) / ts.bn);
+        if (nb >= 512) return t;
+        best = t;
+    }
+    return best;
+}
+
+// Process-wide override used when a caller passes no configuration
+// (set through bli_amd_gemm_set_config; tile -1 restores the heuristic).
+static GemmConfig g_override;
+
+void gemm_run(const GemmDesc& d, const GemmConfig* cfg, hipStream_t st) {
+    if (d.m <= 0 || d.n <= 0) return;
+    if (cfg == nullptr) cfg = &g_override;
+    if (d.k <= 0 || d.alpha == 0.0) {
+        scalm_run(d.dt, d.m, d.n, d.beta, d.c, st);
+        return;
+    }
+    Plan pl = normalise(d, st);
+    GridInfo g{};
+    g.group_m = cfg ? cfg->group_m : 8;
+    g.xcd_remap = cfg ? cfg->xcd_remap : 1;
+    int tile = (cfg && cfg->tile >= 0) ? cfg->tile : gemm_default_tile(d.dt, pl.m, pl.n, pl.k);
+    if (tile < 0 || tile >= tile_count(d.dt)) abort_with(BLIS_INVALID_UKR_ID, "gemm: bad tile");
+    if (d.dt == Dt::F64)
+        launch_tile<double>(tile, make_problem<double>(pl, d.alpha, d.beta), pl.a_k, pl.b_k, g, st);
+    else
+        launch_tile<float>(tile, make_problem<float>(pl, d.alpha, d.beta), pl.a_k, pl.b_k, g, st);
+}
+
+} // namespace blis_amd
+
+extern "C" __attribute__((visibility("default")))
+void bli_amd_gemm_set_config(int tile, int group_m, int xcd_remap, int split_k) {
+    blis_amd::g_override.tile = tile;
+    blis_amd::g_override.group_m = group_m > 0 ? group_m : 8;
+    blis_amd::g_override.xcd_remap = xcd_remap;
+    blis_amd::g_override.split_k = split_k > 0 ? split_k : 1;
+}
+
+extern "C" __attribute__((visibility("default")))
+int bli_amd_gemm_tile_count(int is_double) {
+    return blis_amd::tile_count(is_double ? blis_amd::Dt::F64 : blis_amd::Dt::F32);
+}
+
+extern "C" __attribute__((visibility("default")))
+int bli_amd_gemm_default_tile(int is_double, long long m, long long n, long long k) {
+    return blis_amd::gemm_default_tile(is_double ? blis_amd::Dt::F64 : blis_amd::Dt::F32, m, n, k);
+}

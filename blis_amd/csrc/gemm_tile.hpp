@@ -1,0 +1,366 @@
+// gemm_tile.hpp — device-side building blocks of the gfx950 GEMM.
+//
+// BLIS computes C := beta*C + alpha*A*B with five loops around a register
+// microkernel: jc/pc/ic blocking with packm copies of A and B into cache-
+// resident micro-panels (frame/3/gemm/bli_gemm_blk_var{1,2,3}.c,
+// frame/1m/packm/bli_packm_blk_var1.c), a macro-kernel over MR x NR tiles
+// (frame/3/gemm/bli_gemm_ker_var2.c:37-284) and the microkernel itself
+// (ref_kernels/3/bli_gemm_ref.c:163-317).
+//
+// On gfx950 the same blocking is expressed as:
+//   jc/ic loops   -> the workgroup grid (one BM x BN tile of C per workgroup),
+//                    rasterised in groups and remapped so that neighbouring
+//                    tiles share an XCD's L2;
+//   pc loop       -> the in-kernel k loop over BK-deep slabs;
+//   packm         -> coalesced 16-byte global loads of a BM x BK slab of A and
+//                    a BK x BN slab of B into double-buffered LDS, stored
+//                    k-major ("packed panels") regardless of the source layout;
+//   jr/ir + ukr   -> each wave owns a WM x WN sub-tile built from 16x16x4
+//                    MFMA blocks whose accumulators stay in registers for the
+//                    whole k loop.
+// Inside a wave tile, MFMA block i, row r maps to wave row TM*r + i (and block
+// j, column c to wave column TN*c + j). With that interleave a lane's TM A
+// fragments (and TN B fragments) are contiguous in LDS, so they are fetched
+// as 16-byte vectors, and each lane ends up owning 4*TM consecutive rows of C
+// per column, which it writes back as 16-byte vectors.
+//
+// Semantics kept from the reference microkernel: beta == 0 overwrites C
+// without reading it (bli_gemm_ref.c:257-310); edges m < BM, n < BN and a
+// partial last k slab are handled in-kernel by zero-filled loads and masked
+// stores.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace blis_amd {
+namespace dev {
+
+typedef double d2_t __attribute__((ext_vector_type(2)));
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float  f4_t __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mma;
+template <> struct Mma<double> {
+    using acc_t = d4_t;
+    using vec_t = d2_t;             // 16-byte vector
+    static constexpr int V = 2;     // elements per 16-byte vector
+    __device__ static inline acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mma<float> {
+    using acc_t = f4_t;
+    using vec_t = f4_t;
+    static constexpr int V = 4;
+    __device__ static inline acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+// Workgroup tile BM x BN, k slab BK, wave tile WM x WN.
+template <typename T, int BM_, int BN_, int BK_, int WM_, int WN_>
+struct TileCfg {
+    using M = Mma<T>;
+    static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
+    static constexpr int WGM = BM / WM, WGN = BN / WN;
+    static constexpr int NW = WGM * WGN;          // waves per workgroup
+    static constexpr int NT = 64 * NW;            // threads per workgroup
+    static constexpr int TM = WM / 16, TN = WN / 16;
+    static constexpr int V = M::V;
+    static constexpr int LDA = BM + V;            // LDS row pitch (16 B pad)
+    static constexpr int LDB = BN + V;
+    static constexpr int A_ELEMS = BK * LDA;
+    static constexpr int B_ELEMS = BK * LDB;
+    static constexpr int STAGE = A_ELEMS + B_ELEMS;
+    static constexpr int NVA = (BM * BK / V) / NT;  // 16-byte vectors per thread
+    static constexpr int NVB = (BN * BK / V) / NT;
+    static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile must be 16-aligned");
+    static_assert(BK % 4 == 0 && BK % V == 0, "BK must cover MFMA k and vectors");
+    static_assert((BM * BK / V) % NT == 0 && NVA >= 1, "A slab must split evenly");
+    static_assert((BN * BK / V) % NT == 0 && NVB >= 1, "B slab must split evenly");
+    static_assert(BM % V == 0 && BN % V == 0, "tile must hold whole vectors");
+};
+
+// Parameters of one GEMM problem as seen by a tile routine. Layout flags are
+// template parameters of the routine: A_K means A is contiguous along k
+// (A_eff(i,p) = a[p + i*lda]), otherwise along m (a[i + p*lda]); B_K means B
+// is contiguous along k (B_eff(p,j) = b[p + j*ldb]), otherwise along n
+// (b[j + p*ldb]). C(i,j) = c[i*c_si + j*c_sj].
+template <typename T>
+struct TileProblem {
+    const T* a;
+    const T* b;
+    T*       c;
+    int64_t  m, n, k;
+    int64_t  lda, ldb;
+    int64_t  c_si, c_sj;
+    T        alpha, beta;
+    int      vec_a, vec_b, vec_c;   // 16-byte vector access allowed
+};
+
+// Registers holding one slab of A between its global load and LDS store.
+template <typename T, class TL, bool A_K>
+struct ALoader {
+    using vec_t = typename Mma<T>::vec_t;
+    static constexpr int V = TL::V;
+    vec_t r[TL::NVA];
+
+    __device__ inline void load(const TileProblem<T>& p, int64_t m0, int64_t k0, int tid) {
+#pragma unroll
+        for (int q = 0; q < TL::NVA; ++q) {
+            const int v = tid + q * TL::NT;
+            if constexpr (!A_K) {
+                const int ic = v % (TL::BM / V), kk = v / (TL::BM / V);
+                const int64_t i = m0 + ic * V, pk = k0 + kk;
+                const T* src = p.a + i + pk * p.lda;
+                if (p.vec_a && pk < p.k && i + V <= p.m) {
+                    r[q] = *reinterpret_cast<const vec_t*>(src);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e) {
+                        T x = T(0);
+                        if (pk < p.k && i + e < p.m) x = src[e];
+                        r[q][e] = x;
+                    }
+                }
+            } else {
+                const int kc = v % (TL::BK / V), ii = v / (TL::BK / V);
+                const int64_t i = m0 + ii, pk = k0 + kc * V;
+                const T* src = p.a + pk + i * p.lda;
+                if (p.vec_a && i < p.m && pk + V <= p.k) {
+                    r[q] = *reinterpret_cast<const vec_t*>(src);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e) {
+                        T x = T(0);
+                        if (i < p.m && pk + e < p.k) x = src[e];
+                        r[q][e] = x;
+                    }
+                }
+            }
+        }
+    }
+
+    __device__ inline void store(T* As, int tid) const {
+#pragma unroll
+        for (int q = 0; q < TL::NVA; ++q) {
+            const int v = tid + q * TL::NT;
+            if constexpr (!A_K) {
+                const int ic = v % (TL::BM / V), kk = v / (TL::BM / V);
+                *reinterpret_cast<vec_t*>(As + kk * TL::LDA + ic * V) = r[q];
+            } else {
+                const int kc = v % (TL::BK / V), ii = v / (TL::BK / V);
+#pragma unroll
+                for (int e = 0; e < V; ++e) As[(kc * V + e) * TL::LDA + ii] = r[q][e];
+            }
+        }
+    }
+};
+
+template <typename T, class TL, bool B_K>
+struct BLoader {
+    using vec_t = typename Mma<T>::vec_t;
+    static constexpr int V = TL::V;
+    vec_t r[TL::NVB];
+
+    __device__ inline void load(const TileProblem<T>& p, int64_t n0, int64_t k0, int tid) {
+#pragma unroll
+        for (int q = 0; q < TL::NVB; ++q) {
+            const int v = tid + q * TL::NT;
+            if constexpr (B_K) {
+                const int kc = v % (TL::BK / V), jj = v / (TL::BK / V);
+                const int64_t j = n0 + jj, pk = k0 + kc * V;
+                const T* src = p.b + pk + j * p.ldb;
+                if (p.vec_b && j < p.n && pk + V <= p.k) {
+                    r[q] = *reinterpret_cast<const vec_t*>(src);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e) {
+                        T x = T(0);
+                        if (j < p.n && pk + e < p.k) x = src[e];
+                        r[q][e] = x;
+                    }
+                }
+            } else {
+                const int jc = v % (TL::BN / V), kk = v / (TL::BN / V);
+                const int64_t j = n0 + jc * V, pk = k0 + kk;
+                const T* src = p.b + j + pk * p.ldb;
+                if (p.vec_b && pk < p.k && j + V <= p.n) {
+                    r[q] = *reinterpret_cast<const vec_t*>(src);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e) {
+                        T x = T(0);
+                        if (pk < p.k && j + e < p.n) x = src[e];
+                        r[q][e] = x;
+                    }
+                }
+            }
+        }
+    }
+
+    __device__ inline void store(T* Bs, int tid) const {
+#pragma unroll
+        for (int q = 0; q < TL::NVB; ++q) {
+            const int v = tid + q * TL::NT;
+            if constexpr (B_K) {
+                const int kc = v % (TL::BK / V), jj = v / (TL::BK / V);
+#pragma unroll
+                for (int e = 0; e < V; ++e) Bs[(kc * V + e) * TL::LDB + jj] = r[q][e];
+            } else {
+                const int jc = v % (TL::BN / V), kk = v / (TL::BN / V);
+                *reinterpret_cast<vec_t*>(Bs + kk * TL::LDB + jc * V) = r[q];
+            }
+        }
+    }
+};
+
+// Rank-BK update of the wave's accumulators from one LDS stage.
+template <typename T, class TL>
+__device__ inline void mma_slab(const T* As, const T* Bs,
+                                typename Mma<T>::acc_t (&acc)[TL::TM][TL::TN],
+                                int wm0, int wn0, int lane) {
+    using vec_t = typename Mma<T>::vec_t;
+    constexpr int V = TL::V;
+    const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int ks = 0; ks < TL::BK; ks += 4) {
+        const T* ap = As + (ks + lk) * TL::LDA + wm0 + TL::TM * lr;
+        const T* bp = Bs + (ks + lk) * TL::LDB + wn0 + TL::TN * lr;
+        T af[TL::TM], bf[TL::TN];
+        if constexpr (TL::TM % V == 0) {
+#pragma unroll
+            for (int q = 0; q < TL::TM / V; ++q) {
+                vec_t t = *reinterpret_cast<const vec_t*>(ap + q * V);
+#pragma unroll
+                for (int e = 0; e < V; ++e) af[q * V + e] = t[e];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < TL::TM; ++i) af[i] = ap[i];
+        }
+        if constexpr (TL::TN % V == 0) {
+#pragma unroll
+            for (int q = 0; q < TL::TN / V; ++q) {
+                vec_t t = *reinterpret_cast<const vec_t*>(bp + q * V);
+#pragma unroll
+                for (int e = 0; e < V; ++e) bf[q * V + e] = t[e];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < TL::TN; ++j) bf[j] = bp[j];
+        }
+#pragma unroll
+        for (int i = 0; i < TL::TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TL::TN; ++j)
+                acc[i][j] = Mma<T>::mma(af[i], bf[j], acc[i][j]);
+    }
+}
+
+// Write the wave's accumulators: C := alpha*acc + beta*C (beta == 0 does not
+// read C). A lane owns rows row0 + 4*TM*(lane/16) + [0, 4*TM) of column
+// col0 + TN*(lane%16) + j for each j.
+template <typename T, class TL>
+__device__ inline void store_tile(const TileProblem<T>& p,
+                                  const typename Mma<T>::acc_t (&acc)[TL::TM][TL::TN],
+                                  int64_t row0, int64_t col0, int lane) {
+    using vec_t = typename Mma<T>::vec_t;
+    constexpr int V = TL::V;
+    constexpr int R = 4 * TL::TM;
+    const int g = lane >> 4, lc = lane & 15;
+    const int64_t rbase = row0 + (int64_t)R * g;
+    const T alpha = p.alpha, beta = p.beta;
+#pragma unroll
+    for (int j = 0; j < TL::TN; ++j) {
+        const int64_t col = col0 + TL::TN * lc + j;
+        if (col >= p.n) continue;
+        T vals[R];
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int i = 0; i < TL::TM; ++i) vals[TL::TM * e + i] = acc[i][j][e];
+        if (p.vec_c && rbase + R <= p.m && (R % V) == 0) {
+            T* dst = p.c + rbase + col * p.c_sj;
+            if (beta == T(0)) {
+#pragma unroll
+                for (int q = 0; q < R / V; ++q) {
+                    vec_t o;
+#pragma unroll
+                    for (int e = 0; e < V; ++e) o[e] = alpha * vals[q * V + e];
+                    *reinterpret_cast<vec_t*>(dst + q * V) = o;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < R / V; ++q) {
+                    vec_t old = *reinterpret_cast<const vec_t*>(dst + q * V);
+                    vec_t o;
+#pragma unroll
+                    for (int e = 0; e < V; ++e) o[e] = alpha * vals[q * V + e] + beta * old[e];
+                    *reinterpret_cast<vec_t*>(dst + q * V) = o;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int64_t row = rbase + r;
+                if (row < p.m) {
+                    T* dst = p.c + row * p.c_si + col * p.c_sj;
+                    T o = alpha * vals[r];
+                    if (beta != T(0)) o += beta * *dst;
+                    *dst = o;
+                }
+            }
+        }
+    }
+}
+
+// Compute one BM x BN tile (tile coordinates tm, tn) of one problem. smem
+// must hold 2*TL::STAGE elements. The caller synchronises before reusing smem.
+template <typename T, class TL, bool A_K, bool B_K>
+__device__ inline void gemm_tile(const TileProblem<T>& p, int64_t tm, int64_t tn, T* smem) {
+    using acc_t = typename Mma<T>::acc_t;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % TL::WGM, wn = wave / TL::WGM;
+    const int wm0 = wm * TL::WM, wn0 = wn * TL::WN;
+    const int64_t m0 = tm * TL::BM, n0 = tn * TL::BN;
+
+    acc_t acc[TL::TM][TL::TN];
+#pragma unroll
+    for (int i = 0; i < TL::TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TL::TN; ++j) acc[i][j] = acc_t{};
+
+    ALoader<T, TL, A_K> al;
+    BLoader<T, TL, B_K> bl;
+    const int64_t nk = (p.k + TL::BK - 1) / TL::BK;
+
+    al.load(p, m0, 0, tid);
+    bl.load(p, n0, 0, tid);
+    al.store(smem, tid);
+    bl.store(smem + TL::A_ELEMS, tid);
+    __syncthreads();
+
+    for (int64_t s = 0; s < nk; ++s) {
+        T* cur = smem + (s & 1) * TL::STAGE;
+        T* nxt = smem + ((s + 1) & 1) * TL::STAGE;
+        const bool more = (s + 1) < nk;
+        if (more) {
+            al.load(p, m0, (s + 1) * TL::BK, tid);
+            bl.load(p, n0, (s + 1) * TL::BK, tid);
+        }
+        mma_slab<T, TL>(cur, cur + TL::A_ELEMS, acc, wm0, wn0, lane);
+        if (more) {
+            al.store(nxt, tid);
+            bl.store(nxt + TL::A_ELEMS, tid);
+        }
+        __syncthreads();
+    }
+
+    store_tile<T, TL>(p, acc, m0 + wm0, n0 + wn0, lane);
+}
+
+} // namespace dev
+} // namespace blis_amd

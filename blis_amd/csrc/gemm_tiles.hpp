@@ -1,0 +1,65 @@
+// gemm_tiles.hpp — the gfx950 tile table (the GPU analogue of BLIS's
+// per-architecture blocksize table, config/zen3/bli_cntx_init_zen3.c:204-220).
+//
+// Each entry is (index, BM, BN, BK, WM, WN): workgroup tile BM x BN of C, a
+// BK-deep k slab staged through LDS, and waves owning WM x WN sub-tiles made
+// of 16x16x4 MFMA blocks. Entries are ordered from largest to smallest tile;
+// the "heuristic" flag marks the ones the default planner may pick (the rest
+// are reachable through the autotuner and the learned selector).
+#pragma once
+
+#include <cstdint>
+#include <type_traits>
+
+namespace blis_amd {
+
+struct GridInfo {
+    int grid_m, grid_n;   // tiles along m and n
+    int group_m;          // raster group height (tiles)
+    int xcd_remap;        // 1: contiguous logical tiles per XCD
+};
+
+struct TileShape {
+    int bm, bn, bk, wm, wn;
+    int heuristic;
+};
+
+//                  idx  BM   BN  BK  WM  WN
+#define BLIS_AMD_TILES_F64(X) \
+    X(0, 128, 128, 16, 64, 64) \
+    X(1, 128,  64, 16, 64, 32) \
+    X(2,  64, 128, 16, 32, 64) \
+    X(3,  64,  64, 16, 32, 32) \
+    X(4,  32,  32, 16, 16, 16)
+
+#define BLIS_AMD_TILES_F32(X) \
+    X(0, 128, 128, 32, 64, 64) \
+    X(1, 128,  64, 32, 64, 32) \
+    X(2,  64, 128, 32, 32, 64) \
+    X(3,  64,  64, 32, 32, 32) \
+    X(4,  32,  32, 32, 16, 16)
+
+inline const TileShape& tile_shape(Dt dt, int idx) {
+    static const TileShape f64[] = {
+#define BLIS_AMD_ROW(i, BM, BN, BK, WM, WN) { BM, BN, BK, WM, WN, (BM == BN) },
+        BLIS_AMD_TILES_F64(BLIS_AMD_ROW)
+    };
+    static const TileShape f32[] = {
+        BLIS_AMD_TILES_F32(BLIS_AMD_ROW)
+#undef BLIS_AMD_ROW
+    };
+    return dt == Dt::F64 ? f64[idx] : f32[idx];
+}
+
+inline int tile_count(Dt dt) {
+    (void)dt;
+    return 5;
+}
+
+template <typename T> constexpr Dt dt_of() {
+    return std::is_same<T, double>::value ? Dt::F64 : Dt::F32;
+}
+
+int gemm_default_tile(Dt dt, int64_t m, int64_t n, int64_t k);
+
+} // namespace blis_amd

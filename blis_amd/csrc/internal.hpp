@@ -1,0 +1,109 @@
+// internal.hpp — shared host-side declarations of the MI355X BLIS library.
+//
+// Every public entry point (typed, object and BLAS symbols) reduces its
+// arguments to one of the plain descriptors below and hands it to a launch
+// planner (gemm.hip, trsm.hip, level12.hip, batch.hip). The descriptors hold
+// device pointers only; host-pointer staging happens before they are built.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+#include <vector>
+
+#include "../../include/blis.h"
+
+#define BLIS_AMD_HIDDEN __attribute__((visibility("hidden")))
+
+namespace blis_amd {
+
+enum class Dt : int { F32 = 0, F64 = 1 };
+
+inline size_t dt_size(Dt d) { return d == Dt::F64 ? 8 : 4; }
+
+// Strided 2-D view of device memory: element (i, j) lives at
+// ptr + (i*rs + j*cs) * elem_size.
+struct View {
+    void*   ptr = nullptr;
+    int64_t rs = 1;
+    int64_t cs = 1;
+};
+
+// C := beta*C + alpha*opA(A)*opB(B), after transposition has been folded into
+// the strides: A_eff(i,p) = A[i*a.rs + p*a.cs], B_eff(p,j) = B[p*b.rs + j*b.cs].
+struct GemmDesc {
+    Dt      dt = Dt::F64;
+    int64_t m = 0, n = 0, k = 0;
+    View    a, b, c;
+    double  alpha = 1.0, beta = 0.0;
+};
+
+// Tile/schedule configuration chosen by the planner (or the selector).
+struct GemmConfig {
+    int tile = -1;     // index into the tile table (gemm.hip); -1 = heuristic
+    int group_m = 8;   // raster group height in tiles
+    int xcd_remap = 1; // map consecutive logical tiles onto one XCD
+    int split_k = 1;   // number of k partitions (>1 uses a workspace reduction)
+};
+
+// ---- runtime (runtime.cpp) ----
+BLIS_AMD_HIDDEN void        init_once();
+BLIS_AMD_HIDDEN hipStream_t stream();
+BLIS_AMD_HIDDEN int         device_id();
+BLIS_AMD_HIDDEN void        sync_if_needed();
+BLIS_AMD_HIDDEN void*       workspace(size_t bytes, int slot);
+BLIS_AMD_HIDDEN bool        is_device_pointer(const void* p);
+BLIS_AMD_HIDDEN void        check_hip(hipError_t e, const char* what, const char* file, int line);
+BLIS_AMD_HIDDEN void        abort_with(err_t code, const char* where);
+BLIS_AMD_HIDDEN std::mutex& api_mutex();
+
+#define HIP_CHECK(expr) ::blis_amd::check_hip((expr), #expr, __FILE__, __LINE__)
+
+// Host staging of one strided operand. If the pointer is host memory the
+// touched span is copied to device memory (and back on release when it is
+// an output); device pointers pass through unchanged.
+struct Staged {
+    void*       dev = nullptr;   // device address matching the caller's origin
+    void*       host = nullptr;  // original host origin (nullptr if device)
+    void*       buf = nullptr;   // device allocation (nullptr if device)
+    int64_t     lo = 0;          // lowest element offset touched (<= 0)
+    size_t      bytes = 0;
+    bool        writeback = false;
+    ~Staged();
+    Staged() = default;
+    Staged(const Staged&) = delete;
+    Staged& operator=(const Staged&) = delete;
+};
+BLIS_AMD_HIDDEN void stage(Staged& s, const void* p, Dt dt, int64_t m, int64_t n,
+                           int64_t rs, int64_t cs, bool is_output);
+BLIS_AMD_HIDDEN void stage_vec(Staged& s, const void* p, Dt dt, int64_t n,
+                               int64_t inc, bool is_output);
+
+// ---- kernels' launchers ----
+BLIS_AMD_HIDDEN void gemm_run(const GemmDesc& d, const GemmConfig* cfg, hipStream_t st);
+BLIS_AMD_HIDDEN void scalm_run(Dt dt, int64_t m, int64_t n, double beta, View c, hipStream_t st);
+BLIS_AMD_HIDDEN void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag,
+                              int64_t m, int64_t n, double alpha, View a, View b,
+                              hipStream_t st);
+BLIS_AMD_HIDDEN void axpyv_run(Dt dt, int64_t n, double alpha, const void* x, int64_t incx,
+                               void* y, int64_t incy, hipStream_t st);
+BLIS_AMD_HIDDEN void dotv_run(Dt dt, int64_t n, const void* x, int64_t incx,
+                              const void* y, int64_t incy, void* rho_dev, hipStream_t st);
+BLIS_AMD_HIDDEN void gemv_run(Dt dt, trans_t trans, int64_t m, int64_t n, double alpha,
+                              View a, const void* x, int64_t incx, double beta,
+                              void* y, int64_t incy, hipStream_t st);
+BLIS_AMD_HIDDEN void scalv_run(Dt dt, int64_t n, double beta, void* y, int64_t incy,
+                               hipStream_t st);
+
+struct BatchProblem {
+    int64_t m, n, k;
+    View    a, b, c;      // transposition folded into strides, as in GemmDesc
+    double  alpha, beta;
+};
+BLIS_AMD_HIDDEN void gemm_batch_run(Dt dt, const std::vector<BatchProblem>& probs,
+                                    hipStream_t st);
+
+} // namespace blis_amd

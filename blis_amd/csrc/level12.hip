@@ -1,0 +1,427 @@
+// level12.hip — bandwidth-bound level-1v/level-2 kernels for gfx950:
+// axpyv, dotv, gemv (and the scalv used by gemv's early exits).
+//
+// Reference semantics (paths relative to the BLIS tree):
+//   axpyv  y := y + alpha*x             ref_kernels/1/bli_axpyv_ref.c:38-121
+//   dotv   rho := sum x_i*y_i           ref_kernels/1/bli_dotv_ref.c:38-122
+//   gemv   y := beta*y + alpha*op(A)*x  frame/2/bli_l2_tapi.c:43-123 with
+//          var2 (axpyf over column groups, frame/2/gemv/bli_gemv_unf_var2.c)
+//          and var1 (dotxf over row groups, frame/2/gemv/bli_gemv_unf_var1.c)
+//   scalv with beta == 0 stores zeros without reading y.
+//
+// All of these move each byte of their operands once, so the bound is HBM
+// bandwidth. The kernels use 16-byte vector loads where the stride and
+// alignment allow it, grid-stride loops sized to fill all CUs, and, for the
+// reductions (dotv, gemv with many column chunks), a fixed-order two-pass
+// reduction so results are deterministic from run to run.
+#include "internal.hpp"
+
+namespace blis_amd {
+
+namespace {
+
+constexpr int TPB = 256;
+
+template <typename T> struct V16;
+template <> struct V16<double> { typedef double type __attribute__((ext_vector_type(2))); static constexpr int N = 2; };
+template <> struct V16<float>  { typedef float  type __attribute__((ext_vector_type(4))); static constexpr int N = 4; };
+
+inline bool al16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+int blocks_for(int64_t work_items, int per_thread) {
+    int64_t g = (work_items + (int64_t)TPB * per_thread - 1) / ((int64_t)TPB * per_thread);
+    if (g < 1) g = 1;
+    if (g > 4096) g = 4096;
+    return (int)g;
+}
+
+// ---- axpyv -----------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void axpyv_vec_kernel(int64_t n, T alpha, const T* __restrict__ x,
+                                                        T* __restrict__ y) {
+    using vt = typename V16<T>::type;
+    constexpr int V = V16<T>::N;
+    const int64_t nv = n / V;
+    const int64_t stride = (int64_t)gridDim.x * TPB;
+    const vt* xv = reinterpret_cast<const vt*>(x);
+    vt* yv = reinterpret_cast<vt*>(y);
+    int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    // Four independent 16-byte loads in flight per thread.
+    for (; i + 3 * stride < nv; i += 4 * stride) {
+        vt a0 = xv[i], a1 = xv[i + stride], a2 = xv[i + 2 * stride], a3 = xv[i + 3 * stride];
+        vt b0 = yv[i], b1 = yv[i + stride], b2 = yv[i + 2 * stride], b3 = yv[i + 3 * stride];
+        yv[i] = b0 + alpha * a0;
+        yv[i + stride] = b1 + alpha * a1;
+        yv[i + 2 * stride] = b2 + alpha * a2;
+        yv[i + 3 * stride] = b3 + alpha * a3;
+    }
+    for (; i < nv; i += stride) yv[i] = yv[i] + alpha * xv[i];
+    // Tail elements.
+    const int64_t t = nv * V + blockIdx.x * (int64_t)TPB + threadIdx.x;
+    if (blockIdx.x == 0 && t < n) y[t] = y[t] + alpha * x[t];
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void axpyv_gen_kernel(int64_t n, T alpha, const T* x, int64_t incx,
+                                                        T* y, int64_t incy) {
+    for (int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * TPB) {
+        y[i * incy] = y[i * incy] + alpha * x[i * incx];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void scalv_kernel(int64_t n, T beta, T* y, int64_t incy) {
+    for (int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * TPB) {
+        T* p = y + i * incy;
+        *p = (beta == T(0)) ? T(0) : beta * *p;
+    }
+}
+
+// ---- dotv -------------------------------------------------------------------
+
+template <typename T>
+__device__ inline T block_reduce(T v, T* sh) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    T r = T(0);
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+    }
+    return r;
+}
+
+// Pass 1: block b reduces a fixed, contiguous chunk of the vectors.
+template <typename T>
+__global__ __launch_bounds__(TPB) void dotv_partial_kernel(int64_t n, const T* __restrict__ x,
+                                                           int64_t incx, const T* __restrict__ y,
+                                                           int64_t incy, int64_t chunk,
+                                                           int vec, T* __restrict__ part) {
+    __shared__ T sh[TPB / 64];
+    const int64_t lo = blockIdx.x * chunk;
+    const int64_t hi = (lo + chunk < n) ? lo + chunk : n;
+    T acc0 = T(0), acc1 = T(0);
+    if (vec) {
+        using vt = typename V16<T>::type;
+        constexpr int V = V16<T>::N;
+        const vt* xv = reinterpret_cast<const vt*>(x + lo);
+        const vt* yv = reinterpret_cast<const vt*>(y + lo);
+        const int64_t nv = (hi - lo) / V;
+        int64_t i = threadIdx.x;
+        for (; i + TPB < nv; i += 2 * TPB) {
+            vt a0 = xv[i], b0 = yv[i], a1 = xv[i + TPB], b1 = yv[i + TPB];
+#pragma unroll
+            for (int e = 0; e < V; ++e) { acc0 += a0[e] * b0[e]; acc1 += a1[e] * b1[e]; }
+        }
+        for (; i < nv; i += TPB) {
+            vt a0 = xv[i], b0 = yv[i];
+#pragma unroll
+            for (int e = 0; e < V; ++e) acc0 += a0[e] * b0[e];
+        }
+        for (int64_t t = lo + nv * V + threadIdx.x; t < hi; t += TPB) acc1 += x[t] * y[t];
+    } else {
+        for (int64_t t = lo + threadIdx.x; t < hi; t += TPB) acc0 += x[t * incx] * y[t * incy];
+    }
+    const T r = block_reduce(acc0 + acc1, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void dotv_final_kernel(int nparts, const T* __restrict__ part,
+                                                         T* __restrict__ rho) {
+    __shared__ T sh[TPB / 64];
+    T acc = T(0);
+    for (int i = threadIdx.x; i < nparts; i += TPB) acc += part[i];
+    const T r = block_reduce(acc, sh);
+    if (threadIdx.x == 0) *rho = r;
+}
+
+// ---- gemv, A column-stored (rs == 1): y := beta*y + alpha*A*x --------------
+//
+// Workgroup (rb, cb) owns ROWS rows and the columns of chunk cb. Each thread
+// keeps V consecutive rows and streams the chunk's columns with 16-byte
+// loads; x is staged in LDS GEMV_CW entries at a time. With a single chunk
+// the result goes straight to y; otherwise partial sums go to part[cb][row]
+// and a second kernel reduces them in chunk order.
+constexpr int GEMV_CW = 256;
+
+template <typename T, bool VEC>
+__global__ __launch_bounds__(TPB) void gemv_n_kernel(int64_t m, int64_t n, const T* __restrict__ a,
+                                                     int64_t lda, const T* __restrict__ x,
+                                                     int64_t incx, T alpha, T beta, T* y,
+                                                     int64_t incy, int64_t cols_per_chunk,
+                                                     T* __restrict__ part) {
+    constexpr int V = VEC ? V16<T>::N : 1;
+    constexpr int ROWS = TPB * V;
+    __shared__ T xs[GEMV_CW];
+    const int64_t r0 = blockIdx.x * (int64_t)ROWS + threadIdx.x * V;
+    const int64_t cb = blockIdx.y * cols_per_chunk;
+    const int64_t ce = (cb + cols_per_chunk < n) ? cb + cols_per_chunk : n;
+    T acc[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) acc[e] = T(0);
+    for (int64_t c0 = cb; c0 < ce; c0 += GEMV_CW) {
+        const int64_t c1 = (c0 + GEMV_CW < ce) ? c0 + GEMV_CW : ce;
+        __syncthreads();
+        for (int j = threadIdx.x; j < GEMV_CW; j += TPB) {
+            const int64_t col = c0 + j;
+            xs[j] = (col < c1) ? x[col * incx] : T(0);
+        }
+        __syncthreads();
+        if (r0 >= m) continue;
+        if (VEC && r0 + V <= m) {
+            using vt = typename V16<T>::type;
+            const T* p = a + r0 + c0 * lda;
+            int64_t j = c0;
+            for (; j + 3 < c1; j += 4, p += 4 * lda) {
+                vt v0 = *reinterpret_cast<const vt*>(p);
+                vt v1 = *reinterpret_cast<const vt*>(p + lda);
+                vt v2 = *reinterpret_cast<const vt*>(p + 2 * lda);
+                vt v3 = *reinterpret_cast<const vt*>(p + 3 * lda);
+                const T x0 = xs[j - c0], x1 = xs[j - c0 + 1], x2 = xs[j - c0 + 2], x3 = xs[j - c0 + 3];
+#pragma unroll
+                for (int e = 0; e < V; ++e)
+                    acc[e] += v0[e] * x0 + v1[e] * x1 + v2[e] * x2 + v3[e] * x3;
+            }
+            for (; j < c1; ++j, p += lda) {
+                vt v0 = *reinterpret_cast<const vt*>(p);
+#pragma unroll
+                for (int e = 0; e < V; ++e) acc[e] += v0[e] * xs[j - c0];
+            }
+        } else {
+            for (int64_t j = c0; j < c1; ++j) {
+#pragma unroll
+                for (int e = 0; e < V; ++e)
+                    if (r0 + e < m) acc[e] += a[r0 + e + j * lda] * xs[j - c0];
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const int64_t r = r0 + e;
+        if (r >= m) continue;
+        if (gridDim.y == 1) {
+            T* py = y + r * incy;
+            T o = alpha * acc[e];
+            if (beta != T(0)) o += beta * *py;
+            *py = o;
+        } else {
+            part[blockIdx.y * m + r] = acc[e];
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void gemv_n_reduce_kernel(int64_t m, int nchunks,
+                                                            const T* __restrict__ part, T alpha,
+                                                            T beta, T* y, int64_t incy) {
+    for (int64_t r = blockIdx.x * (int64_t)TPB + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * TPB) {
+        T s = T(0);
+        for (int c = 0; c < nchunks; ++c) s += part[c * m + r];
+        T* py = y + r * incy;
+        T o = alpha * s;
+        if (beta != T(0)) o += beta * *py;
+        *py = o;
+    }
+}
+
+// ---- gemv, A^T with A column-stored: y_j := beta*y_j + alpha*A(:,j).x ------
+// One wave per column of A (dot product along the contiguous dimension).
+template <typename T, bool VEC>
+__global__ __launch_bounds__(TPB) void gemv_t_kernel(int64_t m, int64_t n, const T* __restrict__ a,
+                                                     int64_t lda, const T* __restrict__ x,
+                                                     int64_t incx, T alpha, T beta, T* y,
+                                                     int64_t incy) {
+    const int lane = threadIdx.x & 63;
+    const int64_t col = blockIdx.x * (int64_t)(TPB / 64) + (threadIdx.x >> 6);
+    if (col >= n) return;
+    const T* p = a + col * lda;
+    T acc0 = T(0), acc1 = T(0);
+    if (VEC) {
+        using vt = typename V16<T>::type;
+        constexpr int V = V16<T>::N;
+        const int64_t nv = m / V;
+        const vt* pv = reinterpret_cast<const vt*>(p);
+        const vt* xv = reinterpret_cast<const vt*>(x);
+        int64_t i = lane;
+        for (; i + 64 < nv; i += 128) {
+            vt a0 = pv[i], x0 = xv[i], a1 = pv[i + 64], x1 = xv[i + 64];
+#pragma unroll
+            for (int e = 0; e < V; ++e) { acc0 += a0[e] * x0[e]; acc1 += a1[e] * x1[e]; }
+        }
+        for (; i < nv; i += 64) {
+            vt a0 = pv[i], x0 = xv[i];
+#pragma unroll
+            for (int e = 0; e < V; ++e) acc0 += a0[e] * x0[e];
+        }
+        for (int64_t t = nv * V + lane; t < m; t += 64) acc1 += p[t] * x[t];
+    } else {
+        for (int64_t t = lane; t < m; t += 64) acc0 += p[t] * x[t * incx];
+    }
+    T v = acc0 + acc1;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) {
+        T* py = y + col * incy;
+        T o = alpha * v;
+        if (beta != T(0)) o += beta * *py;
+        *py = o;
+    }
+}
+
+// General strides: one thread per output element.
+template <typename T>
+__global__ __launch_bounds__(TPB) void gemv_gen_kernel(int64_t my, int64_t nx, const T* a,
+                                                       int64_t s_out, int64_t s_in, const T* x,
+                                                       int64_t incx, T alpha, T beta, T* y,
+                                                       int64_t incy) {
+    for (int64_t r = blockIdx.x * (int64_t)TPB + threadIdx.x; r < my;
+         r += (int64_t)gridDim.x * TPB) {
+        T s = T(0);
+        for (int64_t j = 0; j < nx; ++j) s += a[r * s_out + j * s_in] * x[j * incx];
+        T* py = y + r * incy;
+        T o = alpha * s;
+        if (beta != T(0)) o += beta * *py;
+        *py = o;
+    }
+}
+
+template <typename T>
+void axpyv_t(int64_t n, double alpha, const void* x, int64_t incx, void* y, int64_t incy,
+             hipStream_t st) {
+    constexpr int V = V16<T>::N;
+    const bool vec = incx == 1 && incy == 1 && al16(x) && al16(y);
+    if (vec) {
+        const int g = blocks_for(n / V + 1, 4);
+        hipLaunchKernelGGL(axpyv_vec_kernel<T>, dim3(g), dim3(TPB), 0, st, n, (T)alpha,
+                           (const T*)x, (T*)y);
+    } else {
+        const int g = blocks_for(n, 4);
+        hipLaunchKernelGGL(axpyv_gen_kernel<T>, dim3(g), dim3(TPB), 0, st, n, (T)alpha,
+                           (const T*)x, incx, (T*)y, incy);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void dotv_t(int64_t n, const void* x, int64_t incx, const void* y, int64_t incy, void* rho,
+            hipStream_t st) {
+    constexpr int V = V16<T>::N;
+    // Fixed decomposition: chunk size depends only on n, so the summation
+    // order (and the result) is reproducible.
+    int nparts = (int)((n + 65535) / 65536);
+    if (nparts < 1) nparts = 1;
+    if (nparts > 2048) nparts = 2048;
+    int64_t chunk = (n + nparts - 1) / nparts;
+    chunk = (chunk + 7) / 8 * 8;   // keep chunk starts 64-byte aligned
+    if (chunk < 8) chunk = 8;      // n == 0: one empty chunk writes rho = 0
+    nparts = (int)((n + chunk - 1) / chunk);
+    if (nparts < 1) nparts = 1;
+    const bool vec = incx == 1 && incy == 1 && al16(x) && al16(y) && (chunk % V == 0);
+    T* part = (T*)workspace(sizeof(T) * (size_t)nparts, 6);
+    hipLaunchKernelGGL(dotv_partial_kernel<T>, dim3(nparts), dim3(TPB), 0, st, n, (const T*)x,
+                       incx, (const T*)y, incy, chunk, (int)vec, part);
+    hipLaunchKernelGGL(dotv_final_kernel<T>, dim3(1), dim3(TPB), 0, st, nparts,
+                       (const T*)part, (T*)rho);
+    HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void gemv_t_(trans_t trans, int64_t m, int64_t n, double alpha, View a, const void* x,
+             int64_t incx, double beta, void* y, int64_t incy, hipStream_t st) {
+    constexpr int V = V16<T>::N;
+    const bool t = ((unsigned)trans & BLIS_TRANS_BIT) != 0;
+    // y := beta*y + alpha * op(A) * x with op(A) of size m_y x n_x. Express
+    // op(A) as "rows of length n_x": op(A)(r, j) = a[r*s_out + j*s_in].
+    const int64_t my = t ? n : m, nx = t ? m : n;
+    const int64_t s_out = t ? a.cs : a.rs, s_in = t ? a.rs : a.cs;
+    if (s_out == 1) {
+        // Output index is the contiguous one: column-stored axpyf style.
+        const int64_t lda = s_in;
+        const bool vec = al16(a.ptr) && (lda % V == 0);
+        const int rows_per_block = TPB * (vec ? V : 1);
+        const int64_t gx = (my + rows_per_block - 1) / rows_per_block;
+        // Split the columns into chunks only as far as needed to give the
+        // GPU about 2048 workgroups.
+        const int64_t max_chunks = (nx + GEMV_CW - 1) / GEMV_CW;
+        int64_t gy = (2048 + gx - 1) / gx;
+        if (gy > max_chunks) gy = max_chunks;
+        if (gy < 1) gy = 1;
+        int64_t cpc = (nx + gy - 1) / gy;
+        cpc = (cpc + GEMV_CW - 1) / GEMV_CW * GEMV_CW;
+        gy = (nx + cpc - 1) / cpc;
+        T* part = nullptr;
+        if (gy > 1) part = (T*)workspace(sizeof(T) * (size_t)gy * my, 5);
+        if (vec)
+            hipLaunchKernelGGL((gemv_n_kernel<T, true>), dim3((unsigned)gx, (unsigned)gy),
+                               dim3(TPB), 0, st, my, nx, (const T*)a.ptr, lda, (const T*)x,
+                               incx, (T)alpha, (T)beta, (T*)y, incy, cpc, part);
+        else
+            hipLaunchKernelGGL((gemv_n_kernel<T, false>), dim3((unsigned)gx, (unsigned)gy),
+                               dim3(TPB), 0, st, my, nx, (const T*)a.ptr, lda, (const T*)x,
+                               incx, (T)alpha, (T)beta, (T*)y, incy, cpc, part);
+        if (gy > 1) {
+            const int g = blocks_for(my, 1);
+            hipLaunchKernelGGL(gemv_n_reduce_kernel<T>, dim3(g), dim3(TPB), 0, st, my,
+                               (int)gy, (const T*)part, (T)alpha, (T)beta, (T*)y, incy);
+        }
+    } else if (s_in == 1) {
+        // Reduction index is contiguous: dotxf style, one wave per output.
+        const int64_t lda = s_out;
+        const bool vec = al16(a.ptr) && (lda % V == 0) && incx == 1 && al16(x);
+        const unsigned g = (unsigned)((my + (TPB / 64) - 1) / (TPB / 64));
+        if (vec)
+            hipLaunchKernelGGL((gemv_t_kernel<T, true>), dim3(g), dim3(TPB), 0, st, nx, my,
+                               (const T*)a.ptr, lda, (const T*)x, incx, (T)alpha, (T)beta,
+                               (T*)y, incy);
+        else
+            hipLaunchKernelGGL((gemv_t_kernel<T, false>), dim3(g), dim3(TPB), 0, st, nx, my,
+                               (const T*)a.ptr, lda, (const T*)x, incx, (T)alpha, (T)beta,
+                               (T*)y, incy);
+    } else {
+        const int g = blocks_for(my, 1);
+        hipLaunchKernelGGL(gemv_gen_kernel<T>, dim3(g), dim3(TPB), 0, st, my, nx,
+                           (const T*)a.ptr, s_out, s_in, (const T*)x, incx, (T)alpha, (T)beta,
+                           (T*)y, incy);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+} // namespace
+
+void axpyv_run(Dt dt, int64_t n, double alpha, const void* x, int64_t incx, void* y,
+               int64_t incy, hipStream_t st) {
+    if (n <= 0) return;
+    if (dt == Dt::F64) axpyv_t<double>(n, alpha, x, incx, y, incy, st);
+    else axpyv_t<float>(n, alpha, x, incx, y, incy, st);
+}
+
+void dotv_run(Dt dt, int64_t n, const void* x, int64_t incx, const void* y, int64_t incy,
+              void* rho, hipStream_t st) {
+    if (dt == Dt::F64) dotv_t<double>(n, x, incx, y, incy, rho, st);
+    else dotv_t<float>(n, x, incx, y, incy, rho, st);
+}
+
+void scalv_run(Dt dt, int64_t n, double beta, void* y, int64_t incy, hipStream_t st) {
+    if (n <= 0) return;
+    const int g = blocks_for(n, 1);
+    if (dt == Dt::F64)
+        hipLaunchKernelGGL(scalv_kernel<double>, dim3(g), dim3(TPB), 0, st, n, beta, (double*)y,
+                           incy);
+    else
+        hipLaunchKernelGGL(scalv_kernel<float>, dim3(g), dim3(TPB), 0, st, n, (float)beta,
+                           (float*)y, incy);
+    HIP_CHECK(hipGetLastError());
+}
+
+void gemv_run(Dt dt, trans_t trans, int64_t m, int64_t n, double alpha, View a, const void* x,
+              int64_t incx, double beta, void* y, int64_t incy, hipStream_t st) {
+    if (dt == Dt::F64) gemv_t_<double>(trans, m, n, alpha, a, x, incx, beta, y, incy, st);
+    else gemv_t_<float>(trans, m, n, alpha, a, x, incx, beta, y, incy, st);
+}
+
+} // namespace blis_amd

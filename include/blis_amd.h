@@ -1,0 +1,66 @@
+/*
+ * blis_amd.h — extensions of the MI355X BLIS library that have no
+ * counterpart in the reference (BLIS is a synchronous CPU library with no
+ * notion of streams, device plans or tile selection).
+ *
+ * Include after blis.h.
+ */
+#ifndef BLIS_AMD_H
+#define BLIS_AMD_H
+
+#include "blis.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- execution control -------------------------------------------------
+   Run subsequent calls on a caller-provided hipStream_t (NULL restores the
+   library's own non-blocking stream). With sync == 0 calls return as soon as
+   the work is enqueued (only meaningful for device-resident operands); with
+   sync != 0 (the default) every call waits for its results, as BLIS does. */
+void  bli_amd_set_stream( void* hip_stream, int sync );
+void* bli_amd_get_stream( void );
+void  bli_amd_synchronize( void );
+
+/* ---- GEMM configuration ------------------------------------------------
+   Force the tile configuration used by subsequent GEMMs whose caller gives
+   none (tile = index into the gfx950 tile table, -1 = built-in heuristic),
+   the raster group height, the XCD remapping switch and the k split. The
+   arithmetic of a given tile index is fixed, so results depend only on the
+   tile index, never on how it was chosen. */
+void bli_amd_gemm_set_config( int tile, int group_m, int xcd_remap, int split_k );
+int  bli_amd_gemm_tile_count( int is_double );
+int  bli_amd_gemm_default_tile( int is_double, long long m, long long n, long long k );
+
+/* ---- batched GEMM plans ------------------------------------------------
+   A plan holds a batch (same arguments as bli_?gemm_batch, operands already
+   in device memory) as device descriptors and an LPT-ordered work list. Each
+   execute runs the whole batch as one persistent kernel; use_graph != 0
+   replays it as a HIP graph captured on first use. The plan keeps the
+   pointers it was built with. */
+void* bli_amd_dgemm_batch_plan_create( const trans_t* transa, const trans_t* transb,
+                                       const dim_t* m, const dim_t* n, const dim_t* k,
+                                       const double* alpha,
+                                       const double* const* a, const inc_t* rs_a, const inc_t* cs_a,
+                                       const double* const* b, const inc_t* rs_b, const inc_t* cs_b,
+                                       const double* beta,
+                                       double* const* c, const inc_t* rs_c, const inc_t* cs_c,
+                                       dim_t group_count, const dim_t* group_size );
+void* bli_amd_sgemm_batch_plan_create( const trans_t* transa, const trans_t* transb,
+                                       const dim_t* m, const dim_t* n, const dim_t* k,
+                                       const float* alpha,
+                                       const float* const* a, const inc_t* rs_a, const inc_t* cs_a,
+                                       const float* const* b, const inc_t* rs_b, const inc_t* cs_b,
+                                       const float* beta,
+                                       float* const* c, const inc_t* rs_c, const inc_t* cs_c,
+                                       dim_t group_count, const dim_t* group_size );
+void  bli_amd_gemm_batch_plan_execute( void* plan, int use_graph );
+int   bli_amd_gemm_batch_plan_work_items( void* plan );
+void  bli_amd_gemm_batch_plan_destroy( void* plan );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BLIS_AMD_H */

@@ -1,0 +1,133 @@
+"""The C-ABI surface of the built library (CPU only, no device calls).
+
+* Every function declared in include/*.h is exported by libblis_amd.so.
+* Enum values and obj_t layout equal BLIS's (frame/include/bli_type_defs.h:
+  254-520 for the bit encodings, :1233-1263 for obj_t), checked by compiling
+  a probe against include/blis.h with the system C compiler.
+* The netlib driver build needs only blis.h for its integer types, which is
+  the drop-in property for BLAS callers.
+"""
+from __future__ import annotations
+
+import re
+import subprocess
+import tempfile
+from pathlib import Path
+
+import pytest
+
+from tests.conftest import ROOT
+
+LIB = ROOT / "blis_amd" / "lib" / "libblis_amd.so"
+HEADERS = [ROOT / "include" / "blis.h", ROOT / "include" / "blis_amd.h"]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"static inline[^{]*\{[^}]*\}", "", text)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(\s*(?:const\s+)?[A-Za-z_]", text):
+            name = m.group(1)
+            if name.startswith(("bli_", "bli_amd_")) or name.endswith("_"):
+                if name in ("defined",):
+                    continue
+                names.add(name)
+    return names
+
+
+def exported():
+    out = subprocess.run(["nm", "-D", "--defined-only", str(LIB)], capture_output=True,
+                         text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if line.strip()}
+
+
+def test_library_built():
+    assert LIB.exists(), "run __graft_entry__.build() first"
+
+
+def test_every_declared_symbol_is_exported():
+    decl = declared_functions()
+    assert {"bli_dgemm", "dgemm_", "bli_dgemm_batch", "dgemm_batch_", "bli_gemm", "bli_dtrsm",
+            "dtrsm_", "bli_ddotv", "ddot_", "bli_daxpyv", "daxpy_", "bli_dgemv", "dgemv_",
+            "bli_amd_gemm_batch_plan_execute"} <= decl
+    missing = sorted(decl - exported())
+    assert not missing, missing
+
+
+def test_constants_exported():
+    syms = exported()
+    for c in ("BLIS_ONE", "BLIS_ZERO", "BLIS_MINUS_ONE", "BLIS_TWO", "BLIS_MINUS_TWO"):
+        assert c in syms
+
+
+PROBE = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "blis.h"
+int main(void) {
+  printf("%d %d %d %d\n", BLIS_NO_TRANSPOSE, BLIS_TRANSPOSE, BLIS_CONJ_NO_TRANSPOSE, BLIS_CONJ_TRANSPOSE);
+  printf("%d %d %d %d\n", BLIS_ZEROS, BLIS_LOWER, BLIS_UPPER, BLIS_DENSE);
+  printf("%d %d %d %d\n", BLIS_LEFT, BLIS_RIGHT, BLIS_NONUNIT_DIAG, BLIS_UNIT_DIAG);
+  printf("%d %d %d %d %d %d\n", BLIS_FLOAT, BLIS_DOUBLE, BLIS_SCOMPLEX, BLIS_DCOMPLEX, BLIS_INT, BLIS_CONSTANT);
+  printf("%d %d %d %d\n", BLIS_GENERAL, BLIS_HERMITIAN, BLIS_SYMMETRIC, BLIS_TRIANGULAR);
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(obj_t), offsetof(obj_t, info),
+         offsetof(obj_t, elem_size), offsetof(obj_t, buffer), offsetof(obj_t, rs),
+         offsetof(obj_t, scalar), offsetof(obj_t, m_padded), offsetof(obj_t, n_panel));
+  printf("%zu %zu %zu\n", sizeof(dim_t), sizeof(inc_t), sizeof(f77_int));
+  return 0;
+}
+"""
+
+# Values from BLIS's definitions (bli_type_defs.h:254-520): trans bit 3,
+# conj bit 4, uplo bits 5-7, unit-diag bit 8, struc bits 20-21.
+EXPECTED = [
+    "0 8 16 24",
+    "0 192 96 224",
+    "0 1 0 256",
+    "0 2 1 3 4 5",
+    "0 1048576 2097152 3145728",
+    # obj_t on LP64: root 8, off 16, dim 16, diag_off 8 -> info at 48;
+    # info2 4 -> elem_size 56; buffer 64; rs 72; cs 80; is 88; scalar 96;
+    # m_padded 112 ... n_panel 152; size 160.
+    "160 48 56 64 72 96 112 152",
+    "8 8 4",
+]
+
+
+def test_enum_and_layout_match_blis():
+    with tempfile.TemporaryDirectory() as td:
+        src = Path(td) / "probe.c"
+        exe = Path(td) / "probe"
+        src.write_text(PROBE)
+        subprocess.run(["gcc", "-std=c99", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)],
+                       check=True)
+        out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    assert out.strip().splitlines() == EXPECTED
+
+
+def test_header_compiles_as_cxx_and_hides_blas_for_netlib():
+    with tempfile.TemporaryDirectory() as td:
+        src = Path(td) / "probe.cc"
+        src.write_text('#include "blis.h"\n#include "blis_amd.h"\nint main(){return 0;}\n')
+        subprocess.run(["g++", "-std=c++11", f"-I{ROOT / 'include'}", str(src), "-o",
+                        str(Path(td) / "p")], check=True)
+        # With BLIS_VIA_BLASTEST the BLAS prototypes are hidden, so a driver
+        # may declare dgemm_ with its own (f2c) signature.
+        src2 = Path(td) / "netlib_like.c"
+        src2.write_text('#define BLIS_VIA_BLASTEST\n#include "blis.h"\n'
+                        'extern int dgemm_(char*, char*, int*, int*, int*, double*, double*, '
+                        'int*, double*, int*, double*, double*, int*, int, int);\n'
+                        'int main(void){return 0;}\n')
+        subprocess.run(["gcc", "-std=c99", f"-I{ROOT / 'include'}", "-c", str(src2), "-o",
+                        str(Path(td) / "n.o")], check=True)
+
+
+def test_python_obj_t_mirror_matches():
+    import ctypes
+    import blis_amd
+    assert ctypes.sizeof(blis_amd.obj_t) == 160
+    assert blis_amd.obj_t.info.offset == 48
+    assert blis_amd.obj_t.buffer.offset == 64
+    assert blis_amd.obj_t.scalar.offset == 96

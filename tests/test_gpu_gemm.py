@@ -1,0 +1,218 @@
+"""GEMM parity on the GPU through the C ABI (bli_?gemm, dgemm_/sgemm_,
+bli_gemm object API) against the CPU oracle.
+
+Bar: normwise error ||C_gpu - C_oracle||_F / (|alpha| ||A||_F ||B||_F +
+|beta| ||C0||_F) <= 4 k u, with u = 2^-53 (d) or 2^-24 (s) — the c*k*u bound
+the GEMM error analysis gives for either result, doubled for the difference
+of two such results. At full size (8192) the oracle is too slow, so the
+size-independent testsuite probe residual (testsuite/src/test_gemm.c:
+303-414) is used instead, against the testsuite's thresholds.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests.conftest import Mat, normwise_gemm_err, say, unit_roundoff
+
+pytestmark = pytest.mark.gpu
+
+TR = {"n": 0x0, "t": 0x8}
+
+
+def _ct(dtype):
+    return C.c_double if np.dtype(dtype) == np.float64 else C.c_float
+
+
+def run_gemm(blis, dtype, ta, tb, m, n, k, alpha, beta, a: Mat, b: Mat, c: Mat):
+    ch = "d" if np.dtype(dtype) == np.float64 else "s"
+    T = _ct(dtype)
+    al, be = T(alpha), T(beta)
+    getattr(blis, f"bli_{ch}gemm")(TR[ta], TR[tb], m, n, k, C.byref(al), a.ptr, a.rs, a.cs,
+                                   b.ptr, b.rs, b.cs, C.byref(be), c.ptr, c.rs, c.cs)
+
+
+def make_operands(dtype, ta, tb, m, n, k, seed, layout="col"):
+    am, an = (k, m) if ta == "t" else (m, k)
+    bm, bn = (n, k) if tb == "t" else (k, n)
+    if layout == "col":
+        a = Mat(am, an, dtype=dtype, seed=seed)
+        b = Mat(bm, bn, dtype=dtype, seed=seed + 1)
+        c = Mat(m, n, dtype=dtype, seed=seed + 2)
+    elif layout == "row":
+        a = Mat(am, an, rs=an, cs=1, dtype=dtype, seed=seed)
+        b = Mat(bm, bn, rs=bn, cs=1, dtype=dtype, seed=seed + 1)
+        c = Mat(m, n, rs=n, cs=1, dtype=dtype, seed=seed + 2)
+    elif layout == "padded":
+        # lda/ldb/ldc padded and odd offsets (misaligned for 16-byte vectors)
+        a = Mat(am, an, rs=1, cs=am + 3, dtype=dtype, seed=seed, off=1)
+        b = Mat(bm, bn, rs=1, cs=bm + 5, dtype=dtype, seed=seed + 1, off=3)
+        c = Mat(m, n, rs=1, cs=m + 1, dtype=dtype, seed=seed + 2, off=1)
+    else:  # general strides: neither unit row nor unit column stride
+        a = Mat(am, an, rs=2, cs=2 * am + 1, dtype=dtype, seed=seed)
+        b = Mat(bm, bn, rs=bn + 2, cs=3, dtype=dtype, seed=seed + 1)
+        c = Mat(m, n, rs=3, cs=3 * m + 2, dtype=dtype, seed=seed + 2)
+    return a, b, c
+
+
+def check_parity(blis, oracle, dtype, ta, tb, m, n, k, alpha, beta, layout="col", seed=1):
+    say(f"gemm {np.dtype(dtype).name} {ta}{tb} m={m} n={n} k={k} alpha={alpha} beta={beta} {layout}")
+    a, b, c = make_operands(dtype, ta, tb, m, n, k, seed, layout)
+    c0 = c.view().copy()
+    for x in (a, b, c):
+        x.to_device()
+    run_gemm(blis, dtype, ta, tb, m, n, k, alpha, beta, a, b, c)
+    got = c.device_view()
+    ref = c0.copy(order="F")
+    oracle.gemm(ta, tb, alpha, a.view(), b.view(), beta, ref)
+    err = normwise_gemm_err(got, ref, a.view(), b.view(), c0, alpha, beta)
+    assert err <= 4 * max(k, 1) * unit_roundoff(dtype), (ta, tb, m, n, k, layout, err)
+    return err
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("ta", ["n", "t"])
+@pytest.mark.parametrize("tb", ["n", "t"])
+def test_gemm_trans_edges(blis, oracle, dtype, ta, tb):
+    for (m, n, k) in [(1, 1, 1), (7, 5, 3), (67, 45, 33), (128, 128, 128), (257, 130, 70),
+                      (33, 300, 17)]:
+        check_parity(blis, oracle, dtype, ta, tb, m, n, k, 0.7, 1.3)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("layout", ["row", "padded", "general"])
+def test_gemm_layouts(blis, oracle, dtype, layout):
+    for ta, tb in [("n", "n"), ("t", "n"), ("n", "t"), ("t", "t")]:
+        check_parity(blis, oracle, dtype, ta, tb, 77, 61, 45, -1.5, 0.25, layout=layout)
+
+
+def test_gemm_config1_dgemm_f77(blis, oracle):
+    """Config 1: DGEMM NN column-major m=n=k=512, alpha=1, beta=0.5, through
+    the BLAS symbol dgemm_ with device pointers."""
+    m = n = k = 512
+    say("dgemm_ config1 512^3")
+    a = Mat(m, k, seed=100).to_device()
+    b = Mat(k, n, seed=101).to_device()
+    c = Mat(m, n, seed=102)
+    c0 = c.view().copy()
+    c.to_device()
+    mi, ni, ki, ld = (C.c_int32(v) for v in (m, n, k, 512))
+    al, be = C.c_double(1.0), C.c_double(0.5)
+    blis.dgemm_(b"N", b"N", C.byref(mi), C.byref(ni), C.byref(ki), C.byref(al), a.ptr,
+                C.byref(ld), b.ptr, C.byref(ld), C.byref(be), c.ptr, C.byref(ld))
+    got = c.device_view()
+    ref = c0.copy(order="F")
+    oracle.gemm("n", "n", 1.0, a.view(), b.view(), 0.5, ref)
+    err = normwise_gemm_err(got, ref, a.view(), b.view(), c0, 1.0, 0.5)
+    say(f"config1 normwise err {err:.3e}")
+    assert err <= 4 * k * unit_roundoff(np.float64)
+
+
+def test_gemm_host_pointers(blis, oracle):
+    """Host (non-device) operands are staged through device memory."""
+    m, n, k = 90, 70, 50
+    say("dgemm host pointers")
+    a, b, c = Mat(m, k, seed=5), Mat(k, n, seed=6), Mat(m, n, seed=7)
+    c0 = c.view().copy()
+    al, be = C.c_double(1.25), C.c_double(-0.5)
+    blis.bli_dgemm(0, 0, m, n, k, C.byref(al), a.hptr, a.rs, a.cs, b.hptr, b.rs, b.cs,
+                   C.byref(be), c.hptr, c.rs, c.cs)
+    ref = c0.copy(order="F")
+    oracle.gemm("n", "n", 1.25, a.view(), b.view(), -0.5, ref)
+    assert normwise_gemm_err(c.view(), ref, a.view(), b.view(), c0, 1.25, -0.5) <= \
+        4 * k * unit_roundoff(np.float64)
+
+
+def test_gemm_beta_zero_nan_and_alpha_zero(blis):
+    m, n, k = 40, 30, 20
+    say("dgemm beta=0 over NaN, alpha=0 with NaN operands")
+    a, b = Mat(m, k, seed=1).to_device(), Mat(k, n, seed=2).to_device()
+    c = Mat(m, n, fill=np.nan).to_device()
+    al, be = C.c_double(1.0), C.c_double(0.0)
+    blis.bli_dgemm(0, 0, m, n, k, C.byref(al), a.ptr, a.rs, a.cs, b.ptr, b.rs, b.cs,
+                   C.byref(be), c.ptr, c.rs, c.cs)
+    got = c.device_view()
+    assert np.isfinite(got).all()
+    np.testing.assert_allclose(got, a.view() @ b.view(), rtol=1e-13, atol=1e-13)
+    # alpha == 0 must not touch A or B: C := beta * C
+    an = Mat(m, k, fill=np.nan).to_device()
+    bn = Mat(k, n, fill=np.nan).to_device()
+    c2 = Mat(m, n, seed=9)
+    c20 = c2.view().copy()
+    c2.to_device()
+    al, be = C.c_double(0.0), C.c_double(3.0)
+    blis.bli_dgemm(0, 0, m, n, k, C.byref(al), an.ptr, an.rs, an.cs, bn.ptr, bn.rs, bn.cs,
+                   C.byref(be), c2.ptr, c2.rs, c2.cs)
+    np.testing.assert_array_equal(c2.device_view(), 3.0 * c20)
+    # k == 0: C := beta * C
+    blis.bli_dgemm(0, 0, m, n, 0, C.byref(C.c_double(1.0)), an.ptr, an.rs, an.cs, bn.ptr,
+                   bn.rs, bn.cs, C.byref(C.c_double(0.5)), c2.ptr, c2.rs, c2.cs)
+    np.testing.assert_array_equal(c2.device_view(), 1.5 * c20)
+
+
+def test_gemm_object_api(blis, oracle):
+    import blis_amd
+    m, n, k = 65, 66, 67
+    say("bli_gemm object API")
+    a, b, c = Mat(k, m, seed=21), Mat(k, n, seed=22), Mat(m, n, seed=23)
+    c0 = c.view().copy()
+    for x in (a, b, c):
+        x.to_device()
+    ao, bo, co, alo, beo = (blis_amd.obj_t() for _ in range(5))
+    blis.bli_obj_create_with_attached_buffer(blis_amd.BLIS_DOUBLE, k, m, a.ptr, a.rs, a.cs, C.byref(ao))
+    blis.bli_obj_create_with_attached_buffer(blis_amd.BLIS_DOUBLE, k, n, b.ptr, b.rs, b.cs, C.byref(bo))
+    blis.bli_obj_create_with_attached_buffer(blis_amd.BLIS_DOUBLE, m, n, c.ptr, c.rs, c.cs, C.byref(co))
+    alpha, beta = C.c_double(2.0), C.c_double(1.2)
+    blis.bli_obj_create_1x1_with_attached_buffer(blis_amd.BLIS_DOUBLE, C.byref(alpha), C.byref(alo))
+    blis.bli_obj_create_1x1_with_attached_buffer(blis_amd.BLIS_DOUBLE, C.byref(beta), C.byref(beo))
+    ao.info = (ao.info & ~0x18) | blis_amd.BLIS_TRANSPOSE      # op(A) = A^T
+    blis.bli_gemm(C.byref(alo), C.byref(ao), C.byref(bo), C.byref(beo), C.byref(co))
+    got = c.device_view()
+    ref = c0.copy(order="F")
+    oracle.gemm("t", "n", 2.0, a.view(), b.view(), 1.2, ref)
+    assert normwise_gemm_err(got, ref, a.view(), b.view(), c0, 2.0, 1.2) <= 4 * k * unit_roundoff(np.float64)
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
+def test_gemm_every_tile(blis, oracle, tile):
+    """Every entry of the tile table computes the same product."""
+    try:
+        for dtype in (np.float64, np.float32):
+            blis.bli_amd_gemm_set_config(tile, 8, 1, 1)
+            for ta, tb in [("n", "n"), ("t", "t")]:
+                check_parity(blis, oracle, dtype, ta, tb, 300, 200, 100, 1.0, 0.5)
+    finally:
+        blis.bli_amd_gemm_set_config(-1, 8, 1, 1)
+
+
+def probe_resid(a, b, c_hat, c0, alpha, beta, ta, tb, seed=0):
+    rng = np.random.default_rng(seed)
+    t = rng.uniform(-1, 1, c_hat.shape[1])
+    A = a.T if ta == "t" else a
+    B = b.T if tb == "t" else b
+    v = c_hat.astype(np.float64) @ t
+    z = alpha * (A.astype(np.float64) @ (B.astype(np.float64) @ t)) + beta * (c0.astype(np.float64) @ t)
+    scale = (abs(alpha) * np.linalg.norm(A.astype(np.float64)) * np.linalg.norm(B.astype(np.float64))
+             + abs(beta) * np.linalg.norm(c0.astype(np.float64))) * np.linalg.norm(t)
+    return np.linalg.norm(v - z) / scale
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(4096, 4096, 4096), (8192, 8192, 256)])
+def test_gemm_large_probe_residual(blis, dtype, shape):
+    """Full-size parity via the size-independent probe residual."""
+    import torch
+    m, n, k = shape
+    for ta, tb in [("n", "n"), ("t", "n"), ("n", "t"), ("t", "t")]:
+        say(f"large gemm {np.dtype(dtype).name} {ta}{tb} {shape}")
+        a, b, c = make_operands(dtype, ta, tb, m, n, k, seed=31)
+        c0 = c.view().copy()
+        for x in (a, b, c):
+            x.to_device()
+        run_gemm(blis, dtype, ta, tb, m, n, k, 1.0, 0.5, a, b, c)
+        r = probe_resid(a.view(), b.view(), c.device_view(), c0, 1.0, 0.5, ta, tb)
+        assert r <= 4 * k * unit_roundoff(dtype), (ta, tb, r)
+        del a, b, c
+        torch.cuda.empty_cache()

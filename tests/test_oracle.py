@@ -1,0 +1,299 @@
+"""Pin the CPU oracle before trusting it (CPU only).
+
+1. The reference's own known-answer drivers: netlib ?blat3 (GEMM, TRSM) and
+   ?blat2 (GEMV), compiled from the reference tree by oracle/Makefile and
+   linked against the oracle's BLAS symbols. They check every trans/side/
+   uplo/diag combination, alpha/beta in {0, 1, 0.7}/{0, 1, 1.3}, edge sizes
+   {0,1,2,3,5,9}, the error exits (xerbla INFO codes) and untouched padding
+   (blastest/src/dblat3.c:4109-4317, dmmch_).
+2. Mathematical identities that fix each function independently of how it is
+   written: identity and rank-1 GEMMs, brute force in extended precision on
+   tiny edge shapes, the BLIS testsuite's probe-vector residual
+   (testsuite/src/test_gemm.c:303-414), TRSM on B := A*X with known X, dot
+   products of constant vectors, axpy closed forms.
+3. Planted mistakes: each checker used above must reject a result with a
+   plausible error (a dropped term, a transposed operand, a wrong sign).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from tests.conftest import ROOT, normwise_gemm_err, unit_roundoff
+
+REF_DIR = ROOT / "oracle" / "_ref"
+NETLIB_IN = ROOT / "tests" / "golden" / "netlib"
+
+
+def _ensure_ref_built():
+    if not all((REF_DIR / f"{d}_oracle.x").exists() for d in ("dblat3", "dblat2", "sblat3", "sblat2")):
+        if not Path("/root/reference").exists():
+            pytest.skip("netlib drivers not built and the reference tree is absent")
+        subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "all", "ref"], check=True)
+
+
+def run_netlib(driver: str, suffix: str) -> str:
+    exe = REF_DIR / f"{driver}_{suffix}.x"
+    with tempfile.TemporaryDirectory() as td:
+        with open(NETLIB_IN / f"{driver}.in") as fin:
+            p = subprocess.run([str(exe)], stdin=fin, cwd=td, capture_output=True, text=True,
+                               timeout=600)
+        out_file = Path(td) / f"out.{driver}"
+        text = out_file.read_text() if out_file.exists() else ""
+    assert p.returncode == 0, p.stdout + p.stderr
+    return text + p.stdout
+
+
+NETLIB_CASES = [
+    ("dblat3", ["DGEMM", "DTRSM"], {"DGEMM": 17496, "DTRSM": 2592}),
+    ("sblat3", ["SGEMM", "STRSM"], {"SGEMM": 17496, "STRSM": 2592}),
+    ("dblat2", ["DGEMV"], {"DGEMV": 3460}),
+    ("sblat2", ["SGEMV"], {"SGEMV": 3460}),
+]
+
+
+def check_netlib_output(text: str, routines, calls):
+    for r in routines:
+        assert f"{r}  PASSED THE TESTS OF ERROR-EXITS" in text, text[-3000:]
+        assert f"{r}  PASSED THE COMPUTATIONAL TESTS ( {calls[r]:5d} CALLS)" in text or \
+            f"{r}  PASSED THE COMPUTATIONAL TESTS ({calls[r]:6d} CALLS)" in text, text[-3000:]
+    for bad in ("FAILED", "SUSPECT", "******* FATAL ERROR"):
+        assert bad not in text, text[-3000:]
+
+
+@pytest.mark.parametrize("driver,routines,calls", NETLIB_CASES)
+def test_netlib_drivers_on_oracle(driver, routines, calls):
+    _ensure_ref_built()
+    check_netlib_output(run_netlib(driver, "oracle"), routines, calls)
+
+
+# ---------------------------------------------------------------- GEMM --
+
+def _rand(rng, shape, dtype):
+    return np.asfortranarray(rng.uniform(-1, 1, shape).astype(dtype))
+
+
+def _exact_gemm(ta, tb, alpha, a, b, beta, c):
+    """Extended-precision triple loop (long double accumulation)."""
+    A = (a.T if ta == "t" else a).astype(np.longdouble)
+    B = (b.T if tb == "t" else b).astype(np.longdouble)
+    out = np.longdouble(alpha) * (A @ B)
+    if beta != 0:
+        out += np.longdouble(beta) * c.astype(np.longdouble)
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("ta", ["n", "t"])
+@pytest.mark.parametrize("tb", ["n", "t"])
+def test_gemm_bruteforce_edges(oracle, dtype, ta, tb):
+    rng = np.random.default_rng(11)
+    u = unit_roundoff(dtype)
+    for (m, n, k) in [(1, 1, 1), (5, 7, 3), (6, 8, 256), (7, 9, 257), (13, 17, 300), (67, 45, 33)]:
+        for alpha, beta in [(1.0, 0.0), (0.7, 1.3), (-2.0, 0.5)]:
+            a = _rand(rng, (k, m) if ta == "t" else (m, k), dtype)
+            b = _rand(rng, (n, k) if tb == "t" else (k, n), dtype)
+            c0 = _rand(rng, (m, n), dtype)
+            c = c0.copy(order="F")
+            oracle.gemm(ta, tb, alpha, a, b, beta, c)
+            ref = _exact_gemm(ta, tb, alpha, a, b, beta, c0)
+            err = normwise_gemm_err(c, np.asarray(ref, dtype=np.float64),
+                                    a, b, c0, alpha, beta)
+            assert err <= 2 * k * u, (m, n, k, alpha, beta, err)
+
+
+def test_gemm_beta_zero_does_not_read_c(oracle):
+    rng = np.random.default_rng(3)
+    a, b = _rand(rng, (9, 5), np.float64), _rand(rng, (5, 11), np.float64)
+    c = np.full((9, 11), np.nan, order="F")
+    oracle.gemm("n", "n", 1.0, a, b, 0.0, c)
+    assert np.isfinite(c).all()
+    np.testing.assert_allclose(c, a @ b, rtol=1e-14, atol=1e-14)
+
+
+def test_gemm_alpha_zero_scales_c(oracle):
+    a = np.full((4, 3), np.nan, order="F")
+    b = np.full((3, 5), np.nan, order="F")
+    c = np.asfortranarray(np.arange(20.0).reshape(4, 5))
+    oracle.gemm("n", "n", 0.0, a, b, 2.0, c)
+    np.testing.assert_array_equal(c, 2.0 * np.arange(20.0).reshape(4, 5))
+
+
+def test_gemm_identity_and_rank1(oracle):
+    rng = np.random.default_rng(5)
+    n = 37
+    b = _rand(rng, (n, 29), np.float64)
+    c0 = _rand(rng, (n, 29), np.float64)
+    c = c0.copy(order="F")
+    oracle.gemm("n", "n", 2.0, np.asfortranarray(np.eye(n)), b, 0.5, c)
+    np.testing.assert_allclose(c, 2.0 * b + 0.5 * c0, rtol=0, atol=4e-16)
+    # rank-1: k = 1 gives exact products
+    x, y = _rand(rng, (40, 1), np.float64), _rand(rng, (1, 50), np.float64)
+    c = np.zeros((40, 50), order="F")
+    oracle.gemm("n", "n", 1.0, x, y, 0.0, c)
+    np.testing.assert_array_equal(c, x * y)
+
+
+def probe_residual_gemm(a, b, c, c0, alpha, beta, ta, tb, rng):
+    """BLIS testsuite probe-vector residual (testsuite/src/test_gemm.c:303-414):
+    v = C t, z = beta C0 t + alpha op(A) (op(B) t), resid = ||v - z||_2."""
+    A = a.T if ta == "t" else a
+    B = b.T if tb == "t" else b
+    t = rng.uniform(-1, 1, c.shape[1])
+    t /= 2.0 ** np.ceil(np.log2(np.abs(t).sum()))
+    v = c.astype(np.float64) @ t
+    z = alpha * (A.astype(np.float64) @ (B.astype(np.float64) @ t)) + beta * (c0.astype(np.float64) @ t)
+    return np.linalg.norm(v - z)
+
+
+def _normalized(rng, shape, dtype):
+    """testsuite randomization: U[-1,1] divided by the power of two >= ||X||_1
+    (testsuite/src/test_libblis.c:2529-2560)."""
+    x = rng.uniform(-1, 1, shape)
+    x /= 2.0 ** np.ceil(np.log2(np.abs(x).sum(axis=0).max()))
+    return np.asfortranarray(x.astype(dtype))
+
+
+@pytest.mark.parametrize("dtype,thresh", [(np.float64, 1e-14), (np.float32, 1e-5)])
+def test_gemm_testsuite_residual(oracle, dtype, thresh):
+    rng = np.random.default_rng(7)
+    for (m, n, k) in [(100, 100, 100), (300, 200, 150)]:
+        for ta in "nt":
+            for tb in "nt":
+                a = _normalized(rng, (k, m) if ta == "t" else (m, k), dtype)
+                b = _normalized(rng, (n, k) if tb == "t" else (k, n), dtype)
+                c0 = _normalized(rng, (m, n), dtype)
+                c = c0.copy(order="F")
+                oracle.gemm(ta, tb, 2.0, a, b, 1.2, c)
+                assert probe_residual_gemm(a, b, c, c0, 2.0, 1.2, ta, tb, rng) < thresh
+
+
+def test_gemm_general_strides_match_contiguous(oracle):
+    rng = np.random.default_rng(9)
+    base = rng.uniform(-1, 1, 4000)
+    a = np.lib.stride_tricks.as_strided(base, shape=(13, 11), strides=(3 * 8, 50 * 8))
+    b = np.asfortranarray(rng.uniform(-1, 1, (11, 7)))
+    c_strided = np.zeros((13, 7))  # row-major C
+    c_col = np.zeros((13, 7), order="F")
+    oracle.gemm("n", "n", 1.0, a, b, 0.0, c_strided)
+    oracle.gemm("n", "n", 1.0, np.asfortranarray(a), b, 0.0, c_col)
+    np.testing.assert_array_equal(c_strided, c_col)
+
+
+# ---------------------------------------------------------------- TRSM --
+
+@pytest.mark.parametrize("side", [0, 1])
+@pytest.mark.parametrize("uplo", [0xC0, 0x60])
+@pytest.mark.parametrize("ta", ["n", "t"])
+@pytest.mark.parametrize("diag", [0, 0x100])
+def test_trsm_known_solution(oracle, side, uplo, ta, diag):
+    """B := op(A) X (or X op(A)) for a known X; the solve must return X."""
+    rng = np.random.default_rng(13)
+    for (m, n) in [(7, 5), (67, 45), (300, 20)]:
+        mn = m if side == 0 else n
+        # Off-diagonal entries of size 1/mn keep the triangle well conditioned
+        # also with a unit diagonal.
+        a = rng.uniform(-1, 1, (mn, mn)) / mn
+        a[np.diag_indices(mn)] = 1.0 + rng.uniform(0, 1, mn)
+        tri = np.tril(a) if uplo == 0xC0 else np.triu(a)
+        if diag == 0x100:
+            np.fill_diagonal(tri, 1.0)
+        op = tri.T if ta == "t" else tri
+        x = rng.uniform(-1, 1, (m, n))
+        alpha = 2.0
+        b = (op @ x if side == 0 else x @ op) / alpha
+        b = np.asfortranarray(b)
+        # The other triangle (and a unit diagonal) must not be read.
+        a_in = a.copy()
+        if uplo == 0xC0:
+            a_in[np.triu_indices(mn, 1)] = np.nan
+        else:
+            a_in[np.tril_indices(mn, -1)] = np.nan
+        if diag == 0x100:
+            a_in[np.diag_indices(mn)] = np.nan
+        oracle.trsm(side, uplo, ta, diag, alpha, np.asfortranarray(a_in), b)
+        np.testing.assert_allclose(b, x, rtol=0, atol=1e-12)
+
+
+def residual_trsm(a_tri, x, b0, alpha, side):
+    """testsuite/src/test_trsm.c:295-389 style check: ||op(A) X - alpha B||."""
+    lhs = a_tri @ x if side == 0 else x @ a_tri
+    return np.linalg.norm(lhs - alpha * b0) / (np.linalg.norm(a_tri) * np.linalg.norm(x))
+
+
+# ------------------------------------------------------- level-1v / 2 --
+
+def test_dot_axpy_closed_forms(oracle):
+    n = 1000
+    x = np.full(n, 0.5)
+    y = np.full(n, 3.0)
+    assert oracle.dotv(x, y) == 1500.0
+    assert oracle.dotv(x[:0], y[:0], n=0) == 0.0
+    z = y.copy()
+    oracle.axpyv(2.0, x, z)
+    np.testing.assert_array_equal(z, np.full(n, 4.0))
+    z = y.copy()
+    oracle.axpyv(0.0, np.full(n, np.nan), z)
+    np.testing.assert_array_equal(z, y)
+    # strided with a negative increment walks backwards from the origin
+    xs = np.arange(1.0, 11.0)
+    ys = np.zeros(10)
+    oracle.axpyv(1.0, xs[::-1].copy(), ys, incx=1, incy=1)
+    np.testing.assert_array_equal(ys, xs[::-1])
+
+
+@pytest.mark.parametrize("ta", ["n", "t"])
+def test_gemv_matches_extended(oracle, ta):
+    rng = np.random.default_rng(17)
+    a = np.asfortranarray(rng.uniform(-1, 1, (123, 77)))
+    nx = 123 if ta == "t" else 77
+    my = 77 if ta == "t" else 123
+    x = rng.uniform(-1, 1, nx)
+    y0 = rng.uniform(-1, 1, my)
+    y = y0.copy()
+    oracle.gemv(ta, 0.7, a, x, 1.3, y)
+    A = (a.T if ta == "t" else a).astype(np.longdouble)
+    ref = 0.7 * (A @ x.astype(np.longdouble)) + 1.3 * y0
+    np.testing.assert_allclose(y, np.asarray(ref, dtype=np.float64), rtol=0, atol=1e-13)
+    # row-stored A takes the other variant; same result within rounding
+    y2 = y0.copy()
+    oracle.gemv(ta, 0.7, np.ascontiguousarray(a), x, 1.3, y2)
+    np.testing.assert_allclose(y2, y, rtol=0, atol=1e-13)
+
+
+# ---------------------------------------------------- planted mistakes --
+
+def test_checkers_reject_planted_mistakes(oracle):
+    """Each check above must fail on a plausible bug, planted in the oracle's
+    output on the CPU: a dropped k term, a transposed operand, a sign flip."""
+    rng = np.random.default_rng(23)
+    m = n = k = 64
+    u = unit_roundoff(np.float64)
+    a, b, c0 = (_rand(rng, s, np.float64) for s in [(m, k), (k, n), (m, n)])
+    c = c0.copy(order="F")
+    oracle.gemm("n", "n", 1.0, a, b, 0.5, c)
+    ref = np.asarray(_exact_gemm("n", "n", 1.0, a, b, 0.5, c0), dtype=np.float64)
+    assert normwise_gemm_err(c, ref, a, b, c0, 1.0, 0.5) <= 2 * k * u
+    dropped = c - np.outer(a[:, -1], b[-1, :])          # last rank-1 term missing
+    assert normwise_gemm_err(dropped, ref, a, b, c0, 1.0, 0.5) > 1e-3
+    transposed = c0 * 0.5 + a.T @ b                       # A used transposed
+    assert normwise_gemm_err(transposed, ref, a, b, c0, 1.0, 0.5) > 1e-3
+    sign = c0 * 0.5 - a @ b                               # wrong sign on alpha
+    assert normwise_gemm_err(sign, ref, a, b, c0, 1.0, 0.5) > 1e-3
+    # probe residual catches a single corrupted entry
+    bad = c.copy()
+    bad[3, 5] += 1e-9
+    r = np.random.default_rng(1)
+    assert probe_residual_gemm(a, b, bad, c0, 1.0, 0.5, "n", "n", r) > 1e-12
+    # trsm residual catches a solve that skipped the diagonal scaling
+    t = np.tril(rng.uniform(-1, 1, (32, 32))) + 32 * np.eye(32)
+    bb = np.asfortranarray(rng.uniform(-1, 1, (32, 8)))
+    x = bb.copy(order="F")
+    oracle.trsm(0, 0xC0, "n", 0, 1.0, np.asfortranarray(t), x)
+    assert residual_trsm(t, x, bb, 1.0, 0) < 32 * u
+    assert residual_trsm(t, x * 32, bb, 1.0, 0) > 1e-3
