@@ -64,6 +64,7 @@ class Bench:
     def uniform(self, n, dtype):
         t = self.torch.empty(n, dtype=dtype, device="cuda")
         t.uniform_(-1.0, 1.0, generator=self.gen)
+        self.torch.cuda.synchronize()
         return t
 
     def kernel_ms(self, fn, reps):

@@ -127,6 +127,13 @@ def lib():
             raise RuntimeError(
                 f"{LIB_PATH} is missing; build it with `python -c "
                 f"'import __graft_entry__ as g; g.build()'` or `python blis_amd/build.py`")
+        # torch ships its own HIP runtime (libamdhip64 / libhsa-runtime64).
+        # Load it first so that this library binds to the same runtime
+        # instance instead of bringing up a second one in the process.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = C.CDLL(str(LIB_PATH), mode=C.RTLD_LOCAL)
         _sig(_lib)
     return _lib

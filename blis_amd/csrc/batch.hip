@@ -25,7 +25,7 @@ namespace blis_amd {
 
 using namespace dev;
 
-namespace {
+namespace detail_batch {
 
 enum : int32_t {
     F_A_K = 1, F_B_K = 2, F_VEC_A = 4, F_VEC_B = 8, F_VEC_C = 16, F_SMALL = 32
@@ -106,7 +106,8 @@ int cu_count() {
     return n;
 }
 
-} // namespace
+} // namespace detail_batch
+using namespace detail_batch;
 
 // Device-resident batch ready to launch (also the plan object of the
 // bli_amd_?gemm_batch_plan_* extension).

@@ -21,8 +21,8 @@
 // Inside a wave tile, MFMA block i, row r maps to wave row TM*r + i (and block
 // j, column c to wave column TN*c + j). With that interleave a lane's TM A
 // fragments (and TN B fragments) are contiguous in LDS, so they are fetched
-// as 16-byte vectors, and each lane ends up owning 4*TM consecutive rows of C
-// per column, which it writes back as 16-byte vectors.
+// as 16-byte vectors, and each accumulator register of a lane covers TM
+// consecutive rows of C, which are written back as 16-byte vectors.
 //
 // Semantics kept from the reference microkernel: beta == 0 overwrites C
 // without reading it (bli_gemm_ref.c:257-310); edges m < BM, n < BN and a
@@ -41,6 +41,11 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float  f4_t __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Mma;
+// Operand layout (both types, measured on gfx950): lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16]. Accumulator layout differs:
+//   f32 16x16x4: lane l, register r holds D[4*(l/16) + r][l % 16];
+//   f64 16x16x4: lane l, register r holds D[4*r + l/16][l % 16].
+// acc_row(r, g) gives the D row of register r in lane group g = l / 16.
 template <> struct Mma<double> {
     using acc_t = d4_t;
     using vec_t = d2_t;             // 16-byte vector
@@ -48,6 +53,7 @@ template <> struct Mma<double> {
     __device__ static inline acc_t mma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    __device__ static constexpr int acc_row(int r, int g) { return 4 * r + g; }
 };
 template <> struct Mma<float> {
     using acc_t = f4_t;
@@ -56,6 +62,7 @@ template <> struct Mma<float> {
     __device__ static inline acc_t mma(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    __device__ static constexpr int acc_row(int r, int g) { return 4 * g + r; }
 };
 
 // Workgroup tile BM x BN, k slab BK, wave tile WM x WN.
@@ -260,54 +267,51 @@ __device__ inline void mma_slab(const T* As, const T* Bs,
 }
 
 // Write the wave's accumulators: C := alpha*acc + beta*C (beta == 0 does not
-// read C). A lane owns rows row0 + 4*TM*(lane/16) + [0, 4*TM) of column
-// col0 + TN*(lane%16) + j for each j.
+// read C). For accumulator register r, a lane in group g = lane/16 owns the
+// TM consecutive wave rows TM*acc_row(r, g) + [0, TM) of wave column
+// TN*(lane%16) + j, written as 16-byte vectors when they fit.
 template <typename T, class TL>
 __device__ inline void store_tile(const TileProblem<T>& p,
                                   const typename Mma<T>::acc_t (&acc)[TL::TM][TL::TN],
                                   int64_t row0, int64_t col0, int lane) {
     using vec_t = typename Mma<T>::vec_t;
     constexpr int V = TL::V;
-    constexpr int R = 4 * TL::TM;
+    constexpr int TM = TL::TM;
     const int g = lane >> 4, lc = lane & 15;
-    const int64_t rbase = row0 + (int64_t)R * g;
     const T alpha = p.alpha, beta = p.beta;
 #pragma unroll
     for (int j = 0; j < TL::TN; ++j) {
         const int64_t col = col0 + TL::TN * lc + j;
         if (col >= p.n) continue;
-        T vals[R];
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
+        for (int r = 0; r < 4; ++r) {
+            const int64_t rb = row0 + (int64_t)TM * Mma<T>::acc_row(r, g);
+            if constexpr (TM % V == 0) {
+                if (p.vec_c && rb + TM <= p.m) {
+                    T* dst = p.c + rb + col * p.c_sj;
 #pragma unroll
-            for (int i = 0; i < TL::TM; ++i) vals[TL::TM * e + i] = acc[i][j][e];
-        if (p.vec_c && rbase + R <= p.m && (R % V) == 0) {
-            T* dst = p.c + rbase + col * p.c_sj;
-            if (beta == T(0)) {
+                    for (int q = 0; q < TM / V; ++q) {
+                        vec_t o;
+                        if (beta == T(0)) {
 #pragma unroll
-                for (int q = 0; q < R / V; ++q) {
-                    vec_t o;
+                            for (int e = 0; e < V; ++e) o[e] = alpha * acc[q * V + e][j][r];
+                        } else {
+                            const vec_t old = *reinterpret_cast<const vec_t*>(dst + q * V);
 #pragma unroll
-                    for (int e = 0; e < V; ++e) o[e] = alpha * vals[q * V + e];
-                    *reinterpret_cast<vec_t*>(dst + q * V) = o;
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < R / V; ++q) {
-                    vec_t old = *reinterpret_cast<const vec_t*>(dst + q * V);
-                    vec_t o;
-#pragma unroll
-                    for (int e = 0; e < V; ++e) o[e] = alpha * vals[q * V + e] + beta * old[e];
-                    *reinterpret_cast<vec_t*>(dst + q * V) = o;
+                            for (int e = 0; e < V; ++e)
+                                o[e] = alpha * acc[q * V + e][j][r] + beta * old[e];
+                        }
+                        *reinterpret_cast<vec_t*>(dst + q * V) = o;
+                    }
+                    continue;
                 }
             }
-        } else {
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int64_t row = rbase + r;
+            for (int i = 0; i < TM; ++i) {
+                const int64_t row = rb + i;
                 if (row < p.m) {
                     T* dst = p.c + row * p.c_si + col * p.c_sj;
-                    T o = alpha * vals[r];
+                    T o = alpha * acc[i][j][r];
                     if (beta != T(0)) o += beta * *dst;
                     *dst = o;
                 }

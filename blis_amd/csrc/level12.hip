@@ -18,7 +18,7 @@
 
 namespace blis_amd {
 
-namespace {
+namespace detail_level12 {
 
 constexpr int TPB = 256;
 
@@ -391,7 +391,8 @@ void gemv_t_(trans_t trans, int64_t m, int64_t n, double alpha, View a, const vo
     HIP_CHECK(hipGetLastError());
 }
 
-} // namespace
+} // namespace detail_level12
+using namespace detail_level12;
 
 void axpyv_run(Dt dt, int64_t n, double alpha, const void* x, int64_t incx, void* y,
                int64_t incy, hipStream_t st) {

@@ -68,7 +68,10 @@ void init_once() {
             std::abort();
         }
         HIP_CHECK(hipGetDevice(&s.device));
-        HIP_CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+        // A blocking stream: it orders with the legacy default stream, so a
+        // call sees all work the caller queued there before it (BLIS calls
+        // are synchronous and observe all prior writes).
+        HIP_CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamDefault));
     });
 }
 

@@ -25,7 +25,7 @@
 
 namespace blis_amd {
 
-namespace {
+namespace detail_trsm {
 
 constexpr int NB = 64;        // leaf block rows
 constexpr int COLS = 64;      // columns of B per workgroup (one per lane)
@@ -178,7 +178,8 @@ void solve(const Solve& s, int64_t r0, int64_t m, double alpha) {
     }
 }
 
-} // namespace
+} // namespace detail_trsm
+using namespace detail_trsm;
 
 void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag, int64_t m,
               int64_t n, double alpha, View a, View b, hipStream_t st) {

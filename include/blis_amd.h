@@ -16,7 +16,8 @@ extern "C" {
 
 /* ---- execution control -------------------------------------------------
    Run subsequent calls on a caller-provided hipStream_t (NULL restores the
-   library's own non-blocking stream). With sync == 0 calls return as soon as
+   library's own stream, a blocking stream that orders with the legacy
+   default stream). With sync == 0 calls return as soon as
    the work is enqueued (only meaningful for device-resident operands); with
    sync != 0 (the default) every call waits for its results, as BLIS does. */
 void  bli_amd_set_stream( void* hip_stream, int sync );
