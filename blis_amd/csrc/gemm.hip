@@ -211,19 +211,23 @@ static TileProblem<T> make_problem(const Plan& pl, double alpha, double beta) {
 }
 
 int gemm_default_tile(Dt dt, int64_t m, int64_t n, int64_t k) {
-    // Largest tile that still yields at least ~2 workgroups per CU; smaller
-    // tiles otherwise, so that small problems spread over more CUs.
+    // Size heuristic from a sweep of the tile table on MI355X: the best
+    // large-problem tiles are the ones with high wave occupancy (several
+    // waves per SIMD hide LDS and barrier latency); small problems use small
+    // tiles so that enough workgroups exist to fill the 256 CUs.
     (void)k;
-    const int ntiles = tile_count(dt);
-    int best = ntiles - 1;
-    for (int t = 0; t < ntiles; ++t) {
-        const TileShape& ts = tile_shape(dt, t);
-        if (!ts.heuristic) continue;
-        const int64_t nb = ((m + ts.bm - 1) / ts.bm) * ((n + ts.bn - 1) / ts.bn);
-        if (nb >= 512) return t;
-        best = t;
+    auto tiles = [&](int bm, int bn) { return ((m + bm - 1) / bm) * ((n + bn - 1) / bn); };
+    if (dt == Dt::F64) {
+        if (tiles(128, 128) >= 1024) return 13;  // 128x128 (BK 32), 16 waves of 32x32
+        if (tiles(128, 64) >= 1024) return 7;    // 128x64, 8 waves of 32x32
+        if (tiles(64, 64) >= 512) return 3;
+        return 4;
     }
-    return best;
+    if (tiles(256, 128) >= 1024) return 12;      // 256x128 (BK 16), 8 waves of 64x64
+    if (tiles(128, 128) >= 1024) return 8;       // 128x128 (BK 16), 4 waves of 64x64
+    if (tiles(128, 64) >= 512) return 1;
+    if (tiles(64, 64) >= 512) return 3;
+    return 4;
 }
 
 // Process-wide override used when a caller passes no configuration

@@ -149,6 +149,22 @@ struct ALoader {
         }
     }
 
+    // Interior tile, full slab, vector access allowed: straight-line 16-byte
+    // loads with no per-lane conditions (so the loads issue back to back).
+    __device__ inline void load_fast(const TileProblem<T>& p, int64_t m0, int64_t k0, int tid) {
+#pragma unroll
+        for (int q = 0; q < TL::NVA; ++q) {
+            const int v = tid + q * TL::NT;
+            if constexpr (!A_K) {
+                const int ic = v % (TL::BM / V), kk = v / (TL::BM / V);
+                r[q] = *reinterpret_cast<const vec_t*>(p.a + (m0 + ic * V) + (k0 + kk) * p.lda);
+            } else {
+                const int kc = v % (TL::BK / V), ii = v / (TL::BK / V);
+                r[q] = *reinterpret_cast<const vec_t*>(p.a + (k0 + kc * V) + (m0 + ii) * p.lda);
+            }
+        }
+    }
+
     __device__ inline void store(T* As, int tid) const {
 #pragma unroll
         for (int q = 0; q < TL::NVA; ++q) {
@@ -203,6 +219,20 @@ struct BLoader {
                         r[q][e] = x;
                     }
                 }
+            }
+        }
+    }
+
+    __device__ inline void load_fast(const TileProblem<T>& p, int64_t n0, int64_t k0, int tid) {
+#pragma unroll
+        for (int q = 0; q < TL::NVB; ++q) {
+            const int v = tid + q * TL::NT;
+            if constexpr (B_K) {
+                const int kc = v % (TL::BK / V), jj = v / (TL::BK / V);
+                r[q] = *reinterpret_cast<const vec_t*>(p.b + (k0 + kc * V) + (n0 + jj) * p.ldb);
+            } else {
+                const int jc = v % (TL::BN / V), kk = v / (TL::BN / V);
+                r[q] = *reinterpret_cast<const vec_t*>(p.b + (n0 + jc * V) + (k0 + kk) * p.ldb);
             }
         }
     }
@@ -270,7 +300,7 @@ __device__ inline void mma_slab(const T* As, const T* Bs,
 // read C). For accumulator register r, a lane in group g = lane/16 owns the
 // TM consecutive wave rows TM*acc_row(r, g) + [0, TM) of wave column
 // TN*(lane%16) + j, written as 16-byte vectors when they fit.
-template <typename T, class TL>
+template <typename T, class TL, bool INTERIOR>
 __device__ inline void store_tile(const TileProblem<T>& p,
                                   const typename Mma<T>::acc_t (&acc)[TL::TM][TL::TN],
                                   int64_t row0, int64_t col0, int lane) {
@@ -282,12 +312,12 @@ __device__ inline void store_tile(const TileProblem<T>& p,
 #pragma unroll
     for (int j = 0; j < TL::TN; ++j) {
         const int64_t col = col0 + TL::TN * lc + j;
-        if (col >= p.n) continue;
+        if (!INTERIOR && col >= p.n) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int64_t rb = row0 + (int64_t)TM * Mma<T>::acc_row(r, g);
             if constexpr (TM % V == 0) {
-                if (p.vec_c && rb + TM <= p.m) {
+                if (INTERIOR || (p.vec_c && rb + TM <= p.m)) {
                     T* dst = p.c + rb + col * p.c_sj;
 #pragma unroll
                     for (int q = 0; q < TM / V; ++q) {
@@ -322,14 +352,15 @@ __device__ inline void store_tile(const TileProblem<T>& p,
 
 // Compute one BM x BN tile (tile coordinates tm, tn) of one problem. smem
 // must hold 2*TL::STAGE elements. The caller synchronises before reusing smem.
-template <typename T, class TL, bool A_K, bool B_K>
-__device__ inline void gemm_tile(const TileProblem<T>& p, int64_t tm, int64_t tn, T* smem) {
+// FAST: the tile lies inside the matrix and A, B (and C) allow 16-byte access,
+// so every full k slab is loaded without bounds checks.
+template <typename T, class TL, bool A_K, bool B_K, bool FAST>
+__device__ inline void gemm_tile_impl(const TileProblem<T>& p, int64_t m0, int64_t n0, T* smem) {
     using acc_t = typename Mma<T>::acc_t;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave % TL::WGM, wn = wave / TL::WGM;
     const int wm0 = wm * TL::WM, wn0 = wn * TL::WN;
-    const int64_t m0 = tm * TL::BM, n0 = tn * TL::BN;
 
     acc_t acc[TL::TM][TL::TN];
 #pragma unroll
@@ -341,8 +372,17 @@ __device__ inline void gemm_tile(const TileProblem<T>& p, int64_t tm, int64_t tn
     BLoader<T, TL, B_K> bl;
     const int64_t nk = (p.k + TL::BK - 1) / TL::BK;
 
-    al.load(p, m0, 0, tid);
-    bl.load(p, n0, 0, tid);
+    auto load = [&](int64_t k0) {
+        if (FAST && k0 + TL::BK <= p.k) {
+            al.load_fast(p, m0, k0, tid);
+            bl.load_fast(p, n0, k0, tid);
+        } else {
+            al.load(p, m0, k0, tid);
+            bl.load(p, n0, k0, tid);
+        }
+    };
+
+    load(0);
     al.store(smem, tid);
     bl.store(smem + TL::A_ELEMS, tid);
     __syncthreads();
@@ -351,10 +391,7 @@ __device__ inline void gemm_tile(const TileProblem<T>& p, int64_t tm, int64_t tn
         T* cur = smem + (s & 1) * TL::STAGE;
         T* nxt = smem + ((s + 1) & 1) * TL::STAGE;
         const bool more = (s + 1) < nk;
-        if (more) {
-            al.load(p, m0, (s + 1) * TL::BK, tid);
-            bl.load(p, n0, (s + 1) * TL::BK, tid);
-        }
+        if (more) load((s + 1) * TL::BK);
         mma_slab<T, TL>(cur, cur + TL::A_ELEMS, acc, wm0, wn0, lane);
         if (more) {
             al.store(nxt, tid);
@@ -363,7 +400,18 @@ __device__ inline void gemm_tile(const TileProblem<T>& p, int64_t tm, int64_t tn
         __syncthreads();
     }
 
-    store_tile<T, TL>(p, acc, m0 + wm0, n0 + wn0, lane);
+    if (FAST && p.vec_c)
+        store_tile<T, TL, true>(p, acc, m0 + wm0, n0 + wn0, lane);
+    else
+        store_tile<T, TL, false>(p, acc, m0 + wm0, n0 + wn0, lane);
+}
+
+template <typename T, class TL, bool A_K, bool B_K>
+__device__ inline void gemm_tile(const TileProblem<T>& p, int64_t tm, int64_t tn, T* smem) {
+    const int64_t m0 = tm * TL::BM, n0 = tn * TL::BN;
+    const bool fast = p.vec_a && p.vec_b && m0 + TL::BM <= p.m && n0 + TL::BN <= p.n;
+    if (fast) gemm_tile_impl<T, TL, A_K, B_K, true>(p, m0, n0, smem);
+    else gemm_tile_impl<T, TL, A_K, B_K, false>(p, m0, n0, smem);
 }
 
 } // namespace dev

@@ -21,39 +21,54 @@ struct GridInfo {
 
 struct TileShape {
     int bm, bn, bk, wm, wn;
-    int heuristic;
+    int threads;
 };
 
-//                  idx  BM   BN  BK  WM  WN
+//                  idx  BM   BN  BK  WM  WN   (threads = 64 * (BM/WM) * (BN/WN))
 #define BLIS_AMD_TILES_F64(X) \
     X(0, 128, 128, 16, 64, 64) \
     X(1, 128,  64, 16, 64, 32) \
     X(2,  64, 128, 16, 32, 64) \
     X(3,  64,  64, 16, 32, 32) \
-    X(4,  32,  32, 16, 16, 16)
+    X(4,  32,  32, 16, 16, 16) \
+    X(5, 128, 128, 16, 32, 64) \
+    X(6, 128, 128, 16, 64, 32) \
+    X(7, 128,  64, 16, 32, 32) \
+    X(8,  64,  64, 32, 32, 32) \
+    X(9, 128, 128, 32, 32, 64) \
+    X(10, 64, 128, 16, 32, 32) \
+    X(11, 128, 64, 32, 32, 32) \
+    X(12, 128, 128, 16, 32, 32) \
+    X(13, 128, 128, 32, 32, 32)
 
 #define BLIS_AMD_TILES_F32(X) \
     X(0, 128, 128, 32, 64, 64) \
     X(1, 128,  64, 32, 64, 32) \
     X(2,  64, 128, 32, 32, 64) \
     X(3,  64,  64, 32, 32, 32) \
-    X(4,  32,  32, 32, 16, 16)
+    X(4,  32,  32, 32, 16, 16) \
+    X(5, 128, 128, 32, 32, 64) \
+    X(6, 256, 128, 32, 64, 64) \
+    X(7, 128, 256, 32, 64, 64) \
+    X(8, 128, 128, 16, 64, 64) \
+    X(9, 128,  64, 32, 32, 32) \
+    X(10, 64, 128, 32, 32, 32) \
+    X(11, 128, 128, 64, 64, 64) \
+    X(12, 256, 128, 16, 64, 64) \
+    X(13, 128, 128, 16, 32, 64)
+
+#define BLIS_AMD_ROW(i, BM, BN, BK, WM, WN) { BM, BN, BK, WM, WN, 64 * (BM / WM) * (BN / WN) },
 
 inline const TileShape& tile_shape(Dt dt, int idx) {
-    static const TileShape f64[] = {
-#define BLIS_AMD_ROW(i, BM, BN, BK, WM, WN) { BM, BN, BK, WM, WN, (BM == BN) },
-        BLIS_AMD_TILES_F64(BLIS_AMD_ROW)
-    };
-    static const TileShape f32[] = {
-        BLIS_AMD_TILES_F32(BLIS_AMD_ROW)
-#undef BLIS_AMD_ROW
-    };
+    static const TileShape f64[] = { BLIS_AMD_TILES_F64(BLIS_AMD_ROW) };
+    static const TileShape f32[] = { BLIS_AMD_TILES_F32(BLIS_AMD_ROW) };
     return dt == Dt::F64 ? f64[idx] : f32[idx];
 }
 
 inline int tile_count(Dt dt) {
-    (void)dt;
-    return 5;
+#define BLIS_AMD_ONE(i, BM, BN, BK, WM, WN) +1
+    return dt == Dt::F64 ? (0 BLIS_AMD_TILES_F64(BLIS_AMD_ONE)) : (0 BLIS_AMD_TILES_F32(BLIS_AMD_ONE));
+#undef BLIS_AMD_ONE
 }
 
 template <typename T> constexpr Dt dt_of() {
