@@ -211,6 +211,94 @@ def extras(bench: Bench):
     # Batched uneven GEMM (config 4): 10,000 problems, m, n, k log-uniform in [8, 512].
     for ch, key in (("d", "f64"), ("s", "f32")):
         out.update(batch_bench(bench, ch, key, log))
+
+    # Split-precision DGEMM/SGEMM 8192^3 vs native at matched error.
+    out.update(split_bench(bench, log))
+    return out
+
+
+def split_bench(bench, log, size=8192, sample_rows=8):
+    """Split-precision GEMM on u * 2^round(N(0, sigma)) operands.
+
+    For each sigma, the slice count is chosen by the library's policy
+    (nslices = 0). Reported per case:
+      * end-to-end ms (statistics, read-back, slicing, slice GEMM) and
+        effective 2mnk TFLOP/s;
+      * the raw INT8 rate, products * 2mnk, against the 5.0 POP/s INT8 peak;
+      * the normwise error of split and of native GEMM, both against the
+        double-double oracle on `sample_rows` rows of C, measured relative
+        to ||A_rows||_F ||B||_F.
+    """
+    import numpy as np
+    from oracle import pyoracle
+    torch = bench.torch
+    L = bench.L
+    out = {}
+    m = n = k = size
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    cases = (("d", 0.0), ("d", 4.0), ("d", 8.0), ("s", 0.0))
+    for ch, sigma in cases:
+        dt = torch.float64 if ch == "d" else torch.float32
+        T = C.c_double if ch == "d" else C.c_float
+        g = torch.Generator(device="cuda")
+        g.manual_seed(int(sigma) * 17 + (1 if ch == "d" else 2))
+
+        def spread(rows, cols):
+            u = torch.empty(rows * cols, dtype=torch.float64, device="cuda").uniform_(-1, 1, generator=g)
+            if sigma > 0:
+                e = torch.randn(rows * cols, dtype=torch.float64, device="cuda", generator=g)
+                u = torch.ldexp(u, torch.round(e * sigma))
+            return u.to(dt)
+        a, b = spread(m, k), spread(k, n)
+        cs = torch.zeros(m * n, dtype=dt, device="cuda")
+        cn = torch.zeros(m * n, dtype=dt, device="cuda")
+        torch.cuda.synchronize()
+        one, zero = T(1.0), T(0.0)
+        eb = C.c_double()
+        fsplit = getattr(L, f"bli_amd_{ch}gemm_split")
+        used = [0]
+
+        def run_split():
+            used[0] = fsplit(0, 0, m, n, k, C.byref(one), a.data_ptr(), 1, m, b.data_ptr(), 1, k,
+                             C.byref(zero), cs.data_ptr(), 1, m, 0, C.byref(eb))
+        fnat = getattr(L, f"bli_{ch}gemm")
+
+        def run_native():
+            fnat(0, 0, m, n, k, C.byref(one), a.data_ptr(), 1, m, b.data_ptr(), 1, k,
+                 C.byref(zero), cn.data_ptr(), 1, m)
+        ms_split = bench.kernel_ms(run_split, 3)
+        ms_nat = bench.kernel_ms(run_native, 3)
+        s = used[0]
+        # errors on sampled rows vs the double-double oracle
+        rows = np.linspace(0, m - 1, sample_rows).astype(np.int64)
+        A = a.view(k, m).t()  # column-major m x k
+        Ar = np.asfortranarray(A[rows].double().cpu().numpy())
+        Bh = np.asfortranarray(b.view(n, k).t().double().cpu().numpy())
+        ref = np.zeros((sample_rows, n), order="F")
+        pyoracle.gemm_dd_rows("n", "n", 1.0, Ar, Bh, 0.0, ref, 0, sample_rows, nthreads=threads)
+        den = np.linalg.norm(Ar) * np.linalg.norm(Bh)
+        Cs = cs.view(n, m).t()[rows].double().cpu().numpy()
+        Cn = cn.view(n, m).t()[rows].double().cpu().numpy()
+        err_s = float(np.linalg.norm(Cs - ref) / den)
+        err_n = float(np.linalg.norm(Cn - ref) / den)
+        flops = 2.0 * m * n * k
+        tf_s = flops / (ms_split * 1e-3) / 1e12
+        tf_n = flops / (ms_nat * 1e-3) / 1e12
+        prods = s * (s + 1) // 2
+        raw = prods * flops / (ms_split * 1e-3) / 1e12 if s else 0.0
+        key = f"{ch}gemm_split_sigma{int(sigma)}_{m}"
+        out[key] = {"ms": round(ms_split, 4), "TFLOP/s": round(tf_s, 3), "slices": s,
+                    "products": prods, "int8_TOP/s": round(raw, 2),
+                    "int8_frac": round(raw / 5000.0, 4),
+                    "native_ms": round(ms_nat, 4), "native_TFLOP/s": round(tf_n, 3),
+                    "speedup_vs_native": round(ms_nat / ms_split, 3),
+                    "err_split": err_s, "err_native": err_n,
+                    "err_bound_fro": eb.value}
+        log(f"  {key:34s} {ms_split:9.4f} ms  {tf_s:9.3f} TFLOP/s eff  s={s} ({prods} products, "
+            f"{raw:.0f} int8 TOP/s)  native {ms_nat:.3f} ms  err split {err_s:.2e} "
+            f"native {err_n:.2e}")
+        del a, b, cs, cn
+        torch.cuda.empty_cache()
     return out
 
 
@@ -279,6 +367,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-split", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -287,6 +376,9 @@ def main():
 
     import torch
     bench = Bench()
+    if args.only_split:
+        print(json.dumps(split_bench(bench, log)), flush=True)
+        return
     m = n = k = 8192
     log(f"headline: DGEMM NN {m}x{n}x{k}, alpha=1, beta=0.5, warmup {args.warmup}, steps {args.steps}")
     step = bench.gemm_fn("d", 0, 0, m, n, k)

@@ -104,12 +104,16 @@ def _sig(lib):
         s(f"{ch}axpy_", None, PF, PT, P, PF, P, PF)
         s(f"{ch}dot_", T, PF, P, PF, P, PF)
         s(f"{ch}gemm_batch_", None, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P)
+        s(f"bli_amd_{ch}gemm_split", C.c_int, C.c_int, C.c_int, dim_t, dim_t, dim_t, PT, P,
+          inc_t, inc_t, P, inc_t, inc_t, PT, P, inc_t, inc_t, C.c_int, C.POINTER(C.c_double))
     s("bli_amd_gemm_batch_plan_execute", None, P, C.c_int)
     s("bli_amd_gemm_batch_plan_destroy", None, P)
     s("bli_amd_gemm_batch_plan_work_items", C.c_int, P)
     s("bli_amd_set_stream", None, P, C.c_int)
     s("bli_amd_get_stream", P)
     s("bli_amd_synchronize", None)
+    s("bli_amd_gemm_split_slices", C.c_int, C.c_int, C.c_double)
+    s("bli_amd_gemm_split_max_slices", C.c_int, C.c_int)
     s("bli_info_get_version_str", C.c_char_p)
     s("bli_gemm", None, P, P, P, P, P)
     s("bli_trsm", None, C.c_int, P, P, P)

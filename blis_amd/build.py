@@ -46,6 +46,7 @@ SOURCES = [
     "trsm.hip",
     "level12.hip",
     "batch.hip",
+    "split.hip",
 ]
 
 

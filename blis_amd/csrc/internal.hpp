@@ -106,4 +106,14 @@ struct BatchProblem {
 BLIS_AMD_HIDDEN void gemm_batch_run(Dt dt, const std::vector<BatchProblem>& probs,
                                     hipStream_t st);
 
+// ---- split precision (split.hip) ----
+constexpr int SPLIT_MAX_SLICES = 12;
+struct SplitResult {
+    int    slices;      // INT8 slices per operand; 0 = the native path ran
+    double err_bound;   // a-priori Frobenius bound of the truncation error
+};
+BLIS_AMD_HIDDEN SplitResult gemm_split_run(const GemmDesc& d, int nslices, hipStream_t st);
+BLIS_AMD_HIDDEN int split_choose_slices(Dt dt, double gap_bits);
+BLIS_AMD_HIDDEN int split_max_slices(Dt dt);
+
 } // namespace blis_amd

@@ -1,0 +1,522 @@
+// split.hip — split-precision GEMM: C := alpha*op(A)*op(B) + beta*C with the
+// arithmetic done as exact INT8 slice products on the MFMA cores and the
+// slice products summed in FP64.
+//
+// The reference has no such path. Each result must match what its FP64 (FP32)
+// GEMM computes (ref_kernels/3/bli_gemm_ref.c:163-317) to within native
+// precision. The method is the Ozaki scheme with integer slices: Ozaki, Ogita,
+// Oishi, Rump, "Error-free transformations of matrix multiplication by using
+// fast routines of matrix multiplication and its applications", Numer.
+// Algorithms 59 (2012), in the integer form of Ootomo, Ozaki, Yokota, "DGEMM
+// on integer matrix multiplication unit", IJHPCA (2024):
+//
+//   1. row i of op(A) is scaled by 2^-e_i, where e_i is the binary exponent
+//      of its largest magnitude (max = f 2^e_i, f in [0.5, 1)). It is then
+//      split exactly into s signed 7-bit digits:
+//        a_ip 2^-e_i = sum_{t=1..s} A_t[i][p] 2^-7t + r,   |A_t| <= 127,   |r| < 2^-7s.
+//      Column j of op(B) is split the same way (exponent f_j, digits B_u);
+//   2. all products of weight d = t + u <= s + 1 are formed on the INT8 MFMA
+//      units (v_mfma_i32_16x16x64_i8). The products of one weight class are
+//      summed exactly in an int32 accumulator, since (s * 127^2 * k) < 2^31
+//      holds per pass over k. Each k chunk is loaded once, and all of its
+//      slice pairs are multiplied from registers;
+//   3. after the k loop, the class sums are scaled by 2^-7d and added in
+//      FP64, smallest weight first. Then c_ij = alpha 2^(e_i+f_j) sum + beta c_ij.
+//
+// Truncation bound per element:
+//   |c_ij - c~_ij| <= |alpha| 2^(e_i+f_j) k (s+2) 2^-7s.
+// The Frobenius norm of this bound is reported with every call.
+#include "internal.hpp"
+#include "gemm_tiles.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+
+namespace blis_amd {
+
+namespace detail_split {
+
+typedef int i4_t __attribute__((ext_vector_type(4)));
+
+constexpr int KC = 64;          // k per MFMA step (bytes of one slice chunk)
+constexpr int ROWS_ALIGN = 64;  // slice buffers are padded to whole tiles
+constexpr int EXP_NONFINITE = INT_MAX;
+
+// ---- operand statistics --------------------------------------------------
+// Per row of X_eff (rows x K), where X_eff(i,p) = x[i*s_row + p*s_k]:
+//   emax = largest binary exponent (INT_MIN for an all-zero row,
+//          EXP_NONFINITE if any entry is Inf/NaN);
+//   esum, ecnt = sum and count of the exponents of the nonzero entries.
+__global__ void stats_init_kernel(int* emax, long long* esum, int* ecnt, int64_t rows) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        emax[i] = INT_MIN;
+        esum[i] = 0;
+        ecnt[i] = 0;
+    }
+}
+
+template <typename T>
+__device__ inline void stat_one(T v, int& m, long long& sum, int& cnt) {
+    if (v == T(0)) return;
+    if (!isfinite(v)) { m = EXP_NONFINITE; return; }
+    int e;
+    frexp((double)v, &e);
+    m = e > m ? e : m;
+    sum += e;
+    ++cnt;
+}
+
+// row_contig: rows are adjacent in memory (s_row == 1); one thread per row,
+// a k range per grid row. Otherwise one wave per row with lanes along k.
+template <typename T>
+__global__ __launch_bounds__(256) void stats_kernel(const T* __restrict__ x, int64_t rows, int64_t K,
+                                                    int64_t s_row, int64_t s_k, int row_contig,
+                                                    int64_t kchunk, int* emax, long long* esum,
+                                                    int* ecnt) {
+    const int64_t p0 = blockIdx.y * kchunk;
+    const int64_t p1 = (p0 + kchunk < K) ? p0 + kchunk : K;
+    int m = INT_MIN, cnt = 0;
+    long long sum = 0;
+    int64_t i;
+    if (row_contig) {
+        i = blockIdx.x * (int64_t)256 + threadIdx.x;
+        if (i >= rows) return;
+        for (int64_t p = p0; p < p1; ++p) stat_one(x[i * s_row + p * s_k], m, sum, cnt);
+    } else {
+        i = blockIdx.x * (int64_t)4 + (threadIdx.x >> 6);
+        if (i >= rows) return;
+        for (int64_t p = p0 + (threadIdx.x & 63); p < p1; p += 64)
+            stat_one(x[i * s_row + p * s_k], m, sum, cnt);
+    }
+    if (m != INT_MIN) atomicMax(&emax[i], m);
+    if (cnt) {
+        atomicAdd((unsigned long long*)&esum[i], (unsigned long long)sum);
+        atomicAdd(&ecnt[i], cnt);
+    }
+}
+
+// ---- slicing ---------------------------------------------------------------
+// Digits of X_eff in the layout the GEMM reads:
+//   out[((kc * rows_pad + i) * S + t) * 64 + (p % 64)],  kc = p / 64,
+// so a tile of 64 rows at one k chunk is one contiguous block of 64*S*64
+// bytes. Rows >= rows and p >= K are zero. Each thread makes 16 consecutive
+// p of one row; four neighbouring threads cover one row's 64-wide chunk.
+template <typename T>
+__global__ __launch_bounds__(256) void slice_kernel(const T* __restrict__ x, int64_t rows,
+                                                    int64_t rows_pad, int64_t K, int64_t nkc,
+                                                    int64_t s_row, int64_t s_k,
+                                                    const int* __restrict__ emax, int S,
+                                                    signed char* __restrict__ out) {
+    const int64_t total = nkc * rows_pad * 4;
+    for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < total;
+         w += (int64_t)gridDim.x * blockDim.x) {
+        const int sub = (int)(w & 3);
+        const int64_t rest = w >> 2;
+        const int64_t i = rest % rows_pad, kc = rest / rows_pad;
+        double r[16];
+        if (i < rows) {
+            const int em = emax[i];
+            const int e = (em == INT_MIN) ? 0 : em;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int64_t p = kc * KC + sub * 16 + q;
+                const double v = (p < K) ? (double)x[i * s_row + p * s_k] : 0.0;
+                r[q] = ldexp(v, -e);            // |r| < 1, exact
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) r[q] = 0.0;
+        }
+        signed char* dst = out + ((kc * rows_pad + i) * S) * KC + sub * 16;
+        for (int t = 0; t < S; ++t) {
+            union { signed char b[16]; i4_t v; } dg;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const double y = r[q] * 128.0;    // exact
+                const double tr = trunc(y);       // |tr| <= 127
+                dg.b[q] = (signed char)(int)tr;
+                r[q] = y - tr;                    // exact, |r| < 1
+            }
+            *reinterpret_cast<i4_t*>(dst + t * KC) = dg.v;
+        }
+    }
+}
+
+// ---- slice GEMM ------------------------------------------------------------
+// D[x][y] = sum_p X[x][p] Y[y][p] per weight class. X supplies the MFMA's row
+// operand and Y its column operand. A lane's accumulator entries then have y
+// (= lane % 16) as their fastest index, which the host maps to C's unit-stride
+// dimension.
+//
+// Workgroup: 64 x-rows by 64 y-rows, 4 waves as 2 x 2, 32 x 32 per wave (2 x 2
+// MFMA blocks of 16 x 16). Per 64-wide k chunk a wave holds the fragments of
+// all S Y-slices in registers and streams the X-slices, so each slice
+// fragment read from LDS feeds up to S products.
+template <int S>
+struct SliceTile {
+    static constexpr int NT = 256;
+    static constexpr int ROWB = KC * S;                 // bytes of one row's chunk
+    static constexpr int PITCH = ROWB + 16;             // LDS row pitch
+    static constexpr int OPB = 64 * PITCH;              // one operand's LDS tile
+    static constexpr int STAGE = 2 * OPB;
+    static constexpr int NBUF = (2 * STAGE <= 160 * 1024) ? 2 : 1;
+    static constexpr int NLD = 64 * ROWB / 16 / NT;     // 16-byte loads per thread per operand
+    static_assert(NLD == S, "each thread stages S chunks per operand");
+};
+
+struct SliceGemmArgs {
+    const signed char* xs;   // slices of X, rows padded to x_pad
+    const signed char* ys;   // slices of Y, rows padded to y_pad
+    const int* ex;           // exponents (INT_MIN = zero row)
+    const int* ey;
+    void* c;                 // C(x, y) at c + x*c_sx + y*c_sy
+    int64_t c_sx, c_sy;
+    int64_t nx, ny, x_pad, y_pad;
+    int64_t kc0, kc1;        // k chunk range of this pass
+    double alpha, beta;
+};
+
+template <typename T, int S>
+__global__ __launch_bounds__(256) void slice_gemm_kernel(SliceGemmArgs g) {
+    using TL = SliceTile<S>;
+    __shared__ __attribute__((aligned(16))) signed char smem[TL::NBUF * TL::STAGE];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wx = (wave & 1) * 32, wy = (wave >> 1) * 32;
+    const int64_t x0 = (int64_t)blockIdx.x * 64, y0 = (int64_t)blockIdx.y * 64;
+
+    i4_t acc[S][2][2];
+#pragma unroll
+    for (int c = 0; c < S; ++c)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[c][i][j] = i4_t{0, 0, 0, 0};
+
+    i4_t rx[S], ry[S];
+    auto gload = [&](int64_t kc) {
+        const i4_t* gx = reinterpret_cast<const i4_t*>(g.xs + ((kc * g.x_pad + x0) * S) * KC);
+        const i4_t* gy = reinterpret_cast<const i4_t*>(g.ys + ((kc * g.y_pad + y0) * S) * KC);
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            rx[q] = gx[q * TL::NT + tid];
+            ry[q] = gy[q * TL::NT + tid];
+        }
+    };
+    auto sstore = [&](signed char* st) {
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            const int v = q * TL::NT + tid;
+            const int row = v / (TL::ROWB / 16), off = (v % (TL::ROWB / 16)) * 16;
+            *reinterpret_cast<i4_t*>(st + row * TL::PITCH + off) = rx[q];
+            *reinterpret_cast<i4_t*>(st + TL::OPB + row * TL::PITCH + off) = ry[q];
+        }
+    };
+
+    const int lr = lane & 15, lk = (lane >> 4) * 16;
+    gload(g.kc0);
+    sstore(smem);
+    __syncthreads();
+    for (int64_t kc = g.kc0; kc < g.kc1; ++kc) {
+        const int cur = (TL::NBUF == 2) ? (int)((kc - g.kc0) & 1) : 0;
+        const signed char* sx = smem + cur * TL::STAGE;
+        const signed char* sy = sx + TL::OPB;
+        const bool more = kc + 1 < g.kc1;
+        if (more) gload(kc + 1);
+
+        i4_t yf[S][2];
+#pragma unroll
+        for (int u = 0; u < S; ++u)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                yf[u][j] = *reinterpret_cast<const i4_t*>(sy + (wy + 16 * j + lr) * TL::PITCH +
+                                                           u * KC + lk);
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+            i4_t xf[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                xf[i] = *reinterpret_cast<const i4_t*>(sx + (wx + 16 * i + lr) * TL::PITCH +
+                                                        t * KC + lk);
+#pragma unroll
+            for (int u = 0; u + t < S; ++u)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[t + u][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                            xf[i], yf[u][j], acc[t + u][i][j], 0, 0, 0);
+        }
+        if (TL::NBUF == 1) __syncthreads();
+        if (more) sstore(smem + ((TL::NBUF == 2) ? (cur ^ 1) : 0) * TL::STAGE);
+        __syncthreads();
+    }
+
+    // Epilogue: class c (weight d = c + 2) carries 2^-7(c+2).
+    T* C = (T*)g.c;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int64_t y = y0 + wy + 16 * j + lr;
+            if (y >= g.ny) continue;
+            const int fy = g.ey[y] == INT_MIN ? 0 : g.ey[y];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t x = x0 + wx + 16 * i + 4 * (lane >> 4) + r;
+                if (x >= g.nx) continue;
+                double v = 0.0;
+#pragma unroll
+                for (int c = S - 1; c >= 0; --c) v += ldexp((double)acc[c][i][j][r], -7 * (c + 2));
+                const int fx = g.ex[x] == INT_MIN ? 0 : g.ex[x];
+                const double val = g.alpha * ldexp(v, fx + fy);
+                T* dst = C + x * g.c_sx + y * g.c_sy;
+                *dst = (T)(g.beta == 0.0 ? val : val + g.beta * (double)*dst);
+            }
+        }
+}
+
+inline unsigned grid1(int64_t threads) {
+    const int64_t g = (threads + 255) / 256;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 16384));
+}
+
+} // namespace detail_split
+using namespace detail_split;
+
+// Slice count from the operands' exponent range. A row's entries sit on
+// average `gap` bits below the row maximum (the mean of e_max - e over the
+// nonzero entries), and the slices must cover those bits plus the significand
+// that native arithmetic keeps. The constants were fitted with a
+// double-double reference at k = 8192 so that the normwise error stays at or
+// below native GEMM's error for u[-1,1] 2^round(N(0, sigma)) data,
+// sigma = 0, 4, 8.
+int split_choose_slices(Dt dt, double gap_bits) {
+    const double need = (dt == Dt::F64 ? 50.0 : 22.0) + gap_bits;
+    const int s = (int)std::ceil(need / 7.0);
+    return std::max(2, s);
+}
+
+// Slice count at which the INT8 emulation stops paying: past it the slice
+// products take longer than native FP64 (FP32) MFMA. The value comes from
+// gfx950 timings of the two 8192^3 kernels.
+int split_max_slices(Dt dt) { return dt == Dt::F64 ? 9 : 6; }
+
+static void launch_stats(Dt dt, View v, int64_t rows, int64_t K, int64_t s_row, int64_t s_k,
+                         int* emax, long long* esum, int* ecnt, hipStream_t st) {
+    hipLaunchKernelGGL(stats_init_kernel, dim3(grid1(rows)), dim3(256), 0, st, emax, esum, ecnt,
+                       rows);
+    const bool row_contig = (s_row == 1);
+    const int64_t kchunk = 512;
+    const dim3 grid(row_contig ? (unsigned)((rows + 255) / 256) : (unsigned)((rows + 3) / 4),
+                    (unsigned)((K + kchunk - 1) / kchunk));
+    if (dt == Dt::F64)
+        hipLaunchKernelGGL(stats_kernel<double>, grid, dim3(256), 0, st, (const double*)v.ptr,
+                           rows, K, s_row, s_k, (int)row_contig, kchunk, emax, esum, ecnt);
+    else
+        hipLaunchKernelGGL(stats_kernel<float>, grid, dim3(256), 0, st, (const float*)v.ptr,
+                           rows, K, s_row, s_k, (int)row_contig, kchunk, emax, esum, ecnt);
+    HIP_CHECK(hipGetLastError());
+}
+
+static void launch_slice(Dt dt, View v, int64_t rows, int64_t rows_pad, int64_t K, int64_t nkc,
+                         int64_t s_row, int64_t s_k, const int* emax, int S, signed char* out,
+                         hipStream_t st) {
+    const unsigned grid = grid1(nkc * rows_pad * 4);
+    if (dt == Dt::F64)
+        hipLaunchKernelGGL(slice_kernel<double>, dim3(grid), dim3(256), 0, st,
+                           (const double*)v.ptr, rows, rows_pad, K, nkc, s_row, s_k, emax, S, out);
+    else
+        hipLaunchKernelGGL(slice_kernel<float>, dim3(grid), dim3(256), 0, st, (const float*)v.ptr,
+                           rows, rows_pad, K, nkc, s_row, s_k, emax, S, out);
+    HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+static void launch_slice_gemm(int S, const SliceGemmArgs& a, hipStream_t st) {
+    const dim3 grid((unsigned)(a.x_pad / 64), (unsigned)(a.y_pad / 64));
+    switch (S) {
+#define BLIS_AMD_SLICE_CASE(n) \
+    case n: hipLaunchKernelGGL((slice_gemm_kernel<T, n>), grid, dim3(256), 0, st, a); break;
+        BLIS_AMD_SLICE_CASE(2) BLIS_AMD_SLICE_CASE(3) BLIS_AMD_SLICE_CASE(4)
+        BLIS_AMD_SLICE_CASE(5) BLIS_AMD_SLICE_CASE(6) BLIS_AMD_SLICE_CASE(7)
+        BLIS_AMD_SLICE_CASE(8) BLIS_AMD_SLICE_CASE(9) BLIS_AMD_SLICE_CASE(10)
+        BLIS_AMD_SLICE_CASE(11) BLIS_AMD_SLICE_CASE(12)
+#undef BLIS_AMD_SLICE_CASE
+    default: abort_with(BLIS_NOT_YET_IMPLEMENTED, "split gemm: slice count out of range");
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+// Mean over nonzero rows of (e_max - mean exponent).
+static double mean_gap(const int* emax, const long long* esum, const int* ecnt, int64_t rows) {
+    double tot = 0.0;
+    int64_t used = 0;
+    for (int64_t i = 0; i < rows; ++i) {
+        if (ecnt[i] == 0) continue;
+        tot += (double)emax[i] - (double)esum[i] / (double)ecnt[i];
+        ++used;
+    }
+    return used ? tot / (double)used : 0.0;
+}
+
+SplitResult gemm_split_run(const GemmDesc& d, int nslices, hipStream_t st) {
+    SplitResult res{0, 0.0};
+    if (d.m <= 0 || d.n <= 0) return res;
+    if (d.k <= 0 || d.alpha == 0.0) {
+        scalm_run(d.dt, d.m, d.n, d.beta, d.c, st);
+        return res;
+    }
+    if (nslices > SPLIT_MAX_SLICES)
+        abort_with(BLIS_INVALID_DIM_STRIDE_COMBINATION, "split gemm: more than 12 slices");
+
+    // 1. Exponent statistics of op(A)'s rows and op(B)'s columns.
+    const int64_t rows = d.m + d.n;
+    int* emax = (int*)workspace(sizeof(int) * (size_t)rows * 2 + 256, 4);
+    int* ecnt = emax + rows;
+    long long* esum = (long long*)workspace(sizeof(long long) * (size_t)rows + 256, 5);
+    launch_stats(d.dt, d.a, d.m, d.k, d.a.rs, d.a.cs, emax, esum, ecnt, st);
+    launch_stats(d.dt, d.b, d.n, d.k, d.b.cs, d.b.rs, emax + d.m, esum + d.m, ecnt + d.m, st);
+    std::vector<int> h_emax(rows), h_cnt(rows);
+    std::vector<long long> h_sum(rows);
+    HIP_CHECK(hipMemcpyAsync(h_emax.data(), emax, sizeof(int) * rows, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(h_cnt.data(), ecnt, sizeof(int) * rows, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(h_sum.data(), esum, sizeof(long long) * rows, hipMemcpyDeviceToHost,
+                             st));
+    HIP_CHECK(hipStreamSynchronize(st));
+
+    // 2. Slice count; non-finite operands and slice counts past the break-even
+    //    point take the native path (which propagates Inf/NaN as BLIS does).
+    bool finite = true;
+    for (int64_t i = 0; i < rows; ++i) finite &= (h_emax[i] != EXP_NONFINITE);
+    int S = nslices;
+    if (S <= 0) {
+        const double gap = std::max(mean_gap(h_emax.data(), h_sum.data(), h_cnt.data(), d.m),
+                                    mean_gap(h_emax.data() + d.m, h_sum.data() + d.m,
+                                             h_cnt.data() + d.m, d.n));
+        S = split_choose_slices(d.dt, gap);
+        if (S > split_max_slices(d.dt)) S = 0;
+    }
+    if (!finite || S <= 0) {
+        gemm_run(d, nullptr, st);
+        return res;
+    }
+
+    // 3. A-priori bound ||C - C~||_F <= |alpha| k (S+2) 2^-7S
+    //    * sqrt(sum_i 4^e_i) * sqrt(sum_j 4^f_j).
+    auto sum4 = [&](int64_t off, int64_t cnt) {
+        long double s = 0.0L;
+        for (int64_t i = 0; i < cnt; ++i) {
+            const int e = h_emax[off + i];
+            if (e != INT_MIN) s += ldexpl(1.0L, 2 * e);
+        }
+        return s;
+    };
+    res.err_bound = (double)(std::fabs(d.alpha) * (long double)d.k * (S + 2) *
+                             ldexpl(1.0L, -7 * S) * sqrtl(sum4(0, d.m)) * sqrtl(sum4(d.m, d.n)));
+    res.slices = S;
+
+    // 4. Slices. Y is the operand whose index runs along C's unit stride.
+    const int64_t nkc = (d.k + KC - 1) / KC;
+    const int64_t m_pad = (d.m + ROWS_ALIGN - 1) / ROWS_ALIGN * ROWS_ALIGN;
+    const int64_t n_pad = (d.n + ROWS_ALIGN - 1) / ROWS_ALIGN * ROWS_ALIGN;
+    signed char* as = (signed char*)workspace((size_t)nkc * m_pad * S * KC, 2);
+    signed char* bs = (signed char*)workspace((size_t)nkc * n_pad * S * KC, 3);
+    launch_slice(d.dt, d.a, d.m, m_pad, d.k, nkc, d.a.rs, d.a.cs, emax, S, as, st);
+    launch_slice(d.dt, d.b, d.n, n_pad, d.k, nkc, d.b.cs, d.b.rs, emax + d.m, S, bs, st);
+
+    SliceGemmArgs g;
+    const bool c_col = (d.c.rs == 1);   // column-stored C: rows of C run along lanes
+    if (c_col) {
+        g.xs = bs; g.ys = as; g.ex = emax + d.m; g.ey = emax;
+        g.nx = d.n; g.ny = d.m; g.x_pad = n_pad; g.y_pad = m_pad;
+        g.c_sx = d.c.cs; g.c_sy = d.c.rs;
+    } else {
+        g.xs = as; g.ys = bs; g.ex = emax; g.ey = emax + d.m;
+        g.nx = d.m; g.ny = d.n; g.x_pad = m_pad; g.y_pad = n_pad;
+        g.c_sx = d.c.rs; g.c_sy = d.c.cs;
+    }
+    g.c = d.c.ptr;
+    g.alpha = d.alpha;
+
+    // 5. Passes over k short enough that every class sum is exact in int32:
+    //    S * 127^2 * 64 * chunks < 2^31.
+    const int64_t max_chunks =
+        std::max<int64_t>(1, (((int64_t)1 << 31) - 1) / ((int64_t)S * 127 * 127 * KC));
+    for (int64_t k0 = 0; k0 < nkc; k0 += max_chunks) {
+        g.kc0 = k0;
+        g.kc1 = std::min(nkc, k0 + max_chunks);
+        g.beta = (k0 == 0) ? d.beta : 1.0;
+        if (d.dt == Dt::F64) launch_slice_gemm<double>(S, g, st);
+        else launch_slice_gemm<float>(S, g, st);
+    }
+    return res;
+}
+
+} // namespace blis_amd
+
+using namespace blis_amd;
+
+template <typename T>
+static int split_typed(trans_t transa, trans_t transb, dim_t m, dim_t n, dim_t k,
+                       const T* alpha, const T* a, inc_t rs_a, inc_t cs_a,
+                       const T* b, inc_t rs_b, inc_t cs_b, const T* beta,
+                       T* c, inc_t rs_c, inc_t cs_c, int nslices, double* err_bound) {
+    init_once();
+    if (err_bound) *err_bound = 0.0;
+    if (m <= 0 || n <= 0) return 0;
+    const bool ta = ((unsigned)transa & BLIS_TRANS_BIT) != 0;
+    const bool tb = ((unsigned)transb & BLIS_TRANS_BIT) != 0;
+    const Dt dt = dt_of<T>();
+    std::lock_guard<std::mutex> lock(api_mutex());
+    Staged sa, sb, sc;
+    if (k > 0) {
+        stage(sa, a, dt, ta ? k : m, ta ? m : k, rs_a, cs_a, false);
+        stage(sb, b, dt, tb ? n : k, tb ? k : n, rs_b, cs_b, false);
+    }
+    stage(sc, c, dt, m, n, rs_c, cs_c, true);
+    GemmDesc d;
+    d.dt = dt; d.m = m; d.n = n; d.k = k;
+    d.a.ptr = sa.dev; d.a.rs = ta ? cs_a : rs_a; d.a.cs = ta ? rs_a : cs_a;
+    d.b.ptr = sb.dev; d.b.rs = tb ? cs_b : rs_b; d.b.cs = tb ? rs_b : cs_b;
+    d.c.ptr = sc.dev; d.c.rs = rs_c; d.c.cs = cs_c;
+    d.alpha = (double)*alpha;
+    d.beta = (double)*beta;
+    const SplitResult r = gemm_split_run(d, nslices, stream());
+    sync_if_needed();
+    if (err_bound) *err_bound = r.err_bound;
+    return r.slices;
+}
+
+extern "C" {
+
+__attribute__((visibility("default")))
+int bli_amd_dgemm_split(trans_t transa, trans_t transb, dim_t m, dim_t n, dim_t k,
+                        const double* alpha, const double* a, inc_t rs_a, inc_t cs_a,
+                        const double* b, inc_t rs_b, inc_t cs_b, const double* beta,
+                        double* c, inc_t rs_c, inc_t cs_c, int nslices, double* err_bound) {
+    return split_typed<double>(transa, transb, m, n, k, alpha, a, rs_a, cs_a, b, rs_b, cs_b,
+                               beta, c, rs_c, cs_c, nslices, err_bound);
+}
+
+__attribute__((visibility("default")))
+int bli_amd_sgemm_split(trans_t transa, trans_t transb, dim_t m, dim_t n, dim_t k,
+                        const float* alpha, const float* a, inc_t rs_a, inc_t cs_a,
+                        const float* b, inc_t rs_b, inc_t cs_b, const float* beta,
+                        float* c, inc_t rs_c, inc_t cs_c, int nslices, double* err_bound) {
+    return split_typed<float>(transa, transb, m, n, k, alpha, a, rs_a, cs_a, b, rs_b, cs_b,
+                              beta, c, rs_c, cs_c, nslices, err_bound);
+}
+
+__attribute__((visibility("default")))
+int bli_amd_gemm_split_slices(int is_double, double gap_bits) {
+    return split_choose_slices(is_double ? Dt::F64 : Dt::F32, gap_bits);
+}
+
+__attribute__((visibility("default")))
+int bli_amd_gemm_split_max_slices(int is_double) {
+    return split_max_slices(is_double ? Dt::F64 : Dt::F32);
+}
+
+} // extern "C"

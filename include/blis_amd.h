@@ -60,6 +60,36 @@ void  bli_amd_gemm_batch_plan_execute( void* plan, int use_graph );
 int   bli_amd_gemm_batch_plan_work_items( void* plan );
 void  bli_amd_gemm_batch_plan_destroy( void* plan );
 
+/* ---- split-precision GEMM ----------------------------------------------
+   Same arguments and semantics as bli_?gemm (C := beta*C + alpha*op(A)*op(B);
+   beta == 0 does not read C; alpha == 0 or k == 0 scales C by beta). The
+   product is computed from exact INT8 slices of the operands (Ozaki scheme:
+   rows of op(A) and columns of op(B) are scaled by powers of two and split
+   into 7-bit digits, slice products are accumulated exactly in int32 and
+   summed in FP64).
+   nslices > 0 forces the slice count (at most 12). nslices <= 0 chooses it
+   from the operands' exponent range. If the operands hold Inf/NaN, or if the
+   chosen count is past the point where the emulation is slower than native
+   MFMA, the native bli_?gemm path runs instead.
+   Returns the slice count used (0 = native path). If err_bound is non-NULL,
+   it receives an a-priori bound on ||C_split - alpha*op(A)*op(B) - beta*C||_F
+   from the truncation (0 when the native path ran). The call waits for the
+   device to read back the operand statistics. */
+int bli_amd_dgemm_split( trans_t transa, trans_t transb, dim_t m, dim_t n, dim_t k,
+                         const double* alpha, const double* a, inc_t rs_a, inc_t cs_a,
+                         const double* b, inc_t rs_b, inc_t cs_b, const double* beta,
+                         double* c, inc_t rs_c, inc_t cs_c, int nslices, double* err_bound );
+int bli_amd_sgemm_split( trans_t transa, trans_t transb, dim_t m, dim_t n, dim_t k,
+                         const float* alpha, const float* a, inc_t rs_a, inc_t cs_a,
+                         const float* b, inc_t rs_b, inc_t cs_b, const float* beta,
+                         float* c, inc_t rs_c, inc_t cs_c, int nslices, double* err_bound );
+/* Slice count the policy picks for a mean exponent gap (bits between a row's
+   largest exponent and its entries' exponents). */
+int bli_amd_gemm_split_slices( int is_double, double gap_bits );
+/* Largest slice count the automatic policy uses before it falls back to the
+   native path (the break-even point against native MFMA). */
+int bli_amd_gemm_split_max_slices( int is_double );
+
 #ifdef __cplusplus
 }
 #endif
