@@ -47,6 +47,8 @@ SOURCES = [
     "level12.hip",
     "batch.hip",
     "split.hip",
+    "mlp.hip",
+    "select.hip",
 ]
 
 

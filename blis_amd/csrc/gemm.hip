@@ -242,10 +242,21 @@ void gemm_run(const GemmDesc& d, const GemmConfig* cfg, hipStream_t st) {
         return;
     }
     Plan pl = normalise(d, st);
+    GemmConfig picked;
+    if (cfg == &g_override && g_override.tile < 0) {
+        // No configuration given: the selector (heuristic, learned or
+        // exhaustive) picks one from the normalised shape.
+        const int V = d.dt == Dt::F64 ? 2 : 4;
+        const bool va = aligned16(pl.a) && pl.lda % V == 0;
+        const bool vb = aligned16(pl.b) && pl.ldb % V == 0;
+        const bool vc = aligned16(pl.c) && pl.c_si == 1 && pl.c_sj % V == 0;
+        picked = config_of(selector_pick(d.dt, pl.m, pl.n, pl.k, pl.a_k, pl.b_k, va, vb, vc));
+        cfg = &picked;
+    }
     GridInfo g{};
-    g.group_m = cfg ? cfg->group_m : 8;
-    g.xcd_remap = cfg ? cfg->xcd_remap : 1;
-    int tile = (cfg && cfg->tile >= 0) ? cfg->tile : gemm_default_tile(d.dt, pl.m, pl.n, pl.k);
+    g.group_m = cfg->group_m;
+    g.xcd_remap = cfg->xcd_remap;
+    int tile = cfg->tile >= 0 ? cfg->tile : gemm_default_tile(d.dt, pl.m, pl.n, pl.k);
     if (tile < 0 || tile >= tile_count(d.dt)) abort_with(BLIS_INVALID_UKR_ID, "gemm: bad tile");
     if (d.dt == Dt::F64)
         launch_tile<double>(tile, make_problem<double>(pl, d.alpha, d.beta), pl.a_k, pl.b_k, g, st);

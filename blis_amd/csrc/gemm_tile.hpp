@@ -296,6 +296,15 @@ __device__ inline void mma_slab(const T* As, const T* Bs,
     }
 }
 
+// alpha*acc + beta*c as one fixed operation sequence (a product, then a
+// fused multiply-add), used by every store path. Each output element then
+// has the same arithmetic whatever the tile shape or store path, so every
+// tile configuration gives bit-identical results.
+template <typename T>
+__device__ inline T epilogue(T alpha, T acc, T beta, T old) {
+    return fma(beta, old, alpha * acc);
+}
+
 // Write the wave's accumulators: C := alpha*acc + beta*C (beta == 0 does not
 // read C). For accumulator register r, a lane in group g = lane/16 owns the
 // TM consecutive wave rows TM*acc_row(r, g) + [0, TM) of wave column
@@ -329,7 +338,7 @@ __device__ inline void store_tile(const TileProblem<T>& p,
                             const vec_t old = *reinterpret_cast<const vec_t*>(dst + q * V);
 #pragma unroll
                             for (int e = 0; e < V; ++e)
-                                o[e] = alpha * acc[q * V + e][j][r] + beta * old[e];
+                                o[e] = epilogue(alpha, acc[q * V + e][j][r], beta, old[e]);
                         }
                         *reinterpret_cast<vec_t*>(dst + q * V) = o;
                     }
@@ -341,9 +350,8 @@ __device__ inline void store_tile(const TileProblem<T>& p,
                 const int64_t row = rb + i;
                 if (row < p.m) {
                     T* dst = p.c + row * p.c_si + col * p.c_sj;
-                    T o = alpha * acc[i][j][r];
-                    if (beta != T(0)) o += beta * *dst;
-                    *dst = o;
+                    *dst = (beta == T(0)) ? alpha * acc[i][j][r]
+                                          : epilogue(alpha, acc[i][j][r], beta, *dst);
                 }
             }
         }

@@ -106,6 +106,16 @@ struct BatchProblem {
 BLIS_AMD_HIDDEN void gemm_batch_run(Dt dt, const std::vector<BatchProblem>& probs,
                                     hipStream_t st);
 
+// ---- configuration selection (select.hip) ----
+enum { SELECT_HEURISTIC = 0, SELECT_LEARNED = 1, SELECT_EXHAUSTIVE = 2 };
+BLIS_AMD_HIDDEN GemmConfig config_of(int cfg);
+BLIS_AMD_HIDDEN int selector_pick(Dt dt, int64_t m, int64_t n, int64_t k, bool a_k, bool b_k,
+                                  bool va, bool vb, bool vc);
+BLIS_AMD_HIDDEN void gemm_features(Dt dt, int64_t m, int64_t n, int64_t k, bool a_k, bool b_k,
+                                   bool va, bool vb, bool vc, float* f);
+BLIS_AMD_HIDDEN int gemm_autotune(Dt dt, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
+                                  int64_t lda, int64_t ldb, int64_t ldc, float* ms_out);
+
 // ---- split precision (split.hip) ----
 constexpr int SPLIT_MAX_SLICES = 12;
 struct SplitResult {

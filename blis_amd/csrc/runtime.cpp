@@ -28,8 +28,8 @@ struct State {
     bool user_stream = false;
     bool sync = true;
     std::mutex ws_mutex;
-    void* ws[8] = {};
-    size_t ws_bytes[8] = {};
+    void* ws[16] = {};
+    size_t ws_bytes[16] = {};
     std::mutex api;
 };
 
@@ -91,6 +91,7 @@ void sync_if_needed() {
 
 void* workspace(size_t bytes, int slot) {
     State& s = S();
+    if (slot < 0 || slot >= 16) abort_with(BLIS_INVALID_UKR_ID, "workspace slot");
     std::lock_guard<std::mutex> g(s.ws_mutex);
     if (bytes == 0) bytes = 256;
     if (s.ws_bytes[slot] < bytes) {

@@ -48,6 +48,7 @@ constexpr int EXP_NONFINITE = INT_MAX;
 //   emax = largest binary exponent (INT_MIN for an all-zero row,
 //          EXP_NONFINITE if any entry is Inf/NaN);
 //   esum, ecnt = sum and count of the exponents of the nonzero entries.
+// Exponents follow frexp: v = f 2^e with 0.5 <= |f| < 1.
 __global__ void stats_init_kernel(int* emax, long long* esum, int* ecnt, int64_t rows) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -57,90 +58,191 @@ __global__ void stats_init_kernel(int* emax, long long* esum, int* ecnt, int64_t
     }
 }
 
-template <typename T>
-__device__ inline void stat_one(T v, int& m, long long& sum, int& cnt) {
-    if (v == T(0)) return;
-    if (!isfinite(v)) { m = EXP_NONFINITE; return; }
+struct ExpStat {
+    int m = INT_MIN;
+    int cnt = 0;
+    long long sum = 0;
+};
+
+// Exponent of one entry from its bit pattern (subnormals through frexp).
+__device__ inline void stat_one(double v, ExpStat& a) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int be = (int)((b >> 52) & 0x7FF);
     int e;
-    frexp((double)v, &e);
-    m = e > m ? e : m;
-    sum += e;
-    ++cnt;
+    if (be == 0x7FF) { a.m = EXP_NONFINITE; return; }
+    if (be == 0) {
+        if ((b << 1) == 0) return;        // +-0
+        frexp(v, &e);
+    } else {
+        e = be - 1022;
+    }
+    a.m = e > a.m ? e : a.m;
+    a.sum += e;
+    ++a.cnt;
 }
 
-// row_contig: rows are adjacent in memory (s_row == 1); one thread per row,
-// a k range per grid row. Otherwise one wave per row with lanes along k.
+__device__ inline void stat_one(float v, ExpStat& a) {
+    const unsigned b = (unsigned)__float_as_int(v);
+    const int be = (int)((b >> 23) & 0xFF);
+    int e;
+    if (be == 0xFF) { a.m = EXP_NONFINITE; return; }
+    if (be == 0) {
+        if ((b << 1) == 0) return;
+        frexpf(v, &e);
+    } else {
+        e = be - 126;
+    }
+    a.m = e > a.m ? e : a.m;
+    a.sum += e;
+    ++a.cnt;
+}
+
+__device__ inline void stat_merge(ExpStat& a, const ExpStat& b) {
+    a.m = b.m > a.m ? b.m : a.m;
+    a.sum += b.sum;
+    a.cnt += b.cnt;
+}
+
+__device__ inline void stat_commit(const ExpStat& a, int64_t i, int* emax, long long* esum,
+                                   int* ecnt) {
+    if (a.m != INT_MIN) atomicMax(&emax[i], a.m);
+    if (a.cnt) {
+        atomicAdd((unsigned long long*)&esum[i], (unsigned long long)a.sum);
+        atomicAdd(&ecnt[i], a.cnt);
+    }
+}
+
+// Rows adjacent in memory (s_row == 1): lanes own 64 consecutive rows, the
+// four waves interleave over the block's k range (p = p0 + wave + 4j), and
+// the waves' partial statistics are merged through LDS.
 template <typename T>
-__global__ __launch_bounds__(256) void stats_kernel(const T* __restrict__ x, int64_t rows, int64_t K,
-                                                    int64_t s_row, int64_t s_k, int row_contig,
-                                                    int64_t kchunk, int* emax, long long* esum,
-                                                    int* ecnt) {
+__global__ __launch_bounds__(256) void stats_rows_kernel(const T* __restrict__ x, int64_t rows,
+                                                         int64_t K, int64_t s_k, int64_t kchunk,
+                                                         int* emax, long long* esum, int* ecnt) {
+    __shared__ int sm[4][64], sc[4][64];
+    __shared__ long long ss[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i = blockIdx.x * (int64_t)64 + lane;
     const int64_t p0 = blockIdx.y * kchunk;
     const int64_t p1 = (p0 + kchunk < K) ? p0 + kchunk : K;
-    int m = INT_MIN, cnt = 0;
-    long long sum = 0;
-    int64_t i;
-    if (row_contig) {
-        i = blockIdx.x * (int64_t)256 + threadIdx.x;
-        if (i >= rows) return;
-        for (int64_t p = p0; p < p1; ++p) stat_one(x[i * s_row + p * s_k], m, sum, cnt);
-    } else {
-        i = blockIdx.x * (int64_t)4 + (threadIdx.x >> 6);
-        if (i >= rows) return;
-        for (int64_t p = p0 + (threadIdx.x & 63); p < p1; p += 64)
-            stat_one(x[i * s_row + p * s_k], m, sum, cnt);
+    ExpStat a;
+    if (i < rows) {
+        const T* xr = x + i;
+        int64_t p = p0 + wave;
+        for (; p + 28 < p1; p += 32) {
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = xr[(p + 4 * u) * s_k];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) stat_one(v[u], a);
+        }
+        for (; p < p1; p += 4) stat_one(xr[p * s_k], a);
     }
-    if (m != INT_MIN) atomicMax(&emax[i], m);
-    if (cnt) {
-        atomicAdd((unsigned long long*)&esum[i], (unsigned long long)sum);
-        atomicAdd(&ecnt[i], cnt);
+    sm[wave][lane] = a.m;
+    sc[wave][lane] = a.cnt;
+    ss[wave][lane] = a.sum;
+    __syncthreads();
+    if (wave == 0 && i < rows) {
+        for (int w = 1; w < 4; ++w) {
+            ExpStat b;
+            b.m = sm[w][lane];
+            b.cnt = sc[w][lane];
+            b.sum = ss[w][lane];
+            stat_merge(a, b);
+        }
+        stat_commit(a, i, emax, esum, ecnt);
     }
+}
+
+// Any other layout: one wave per row, lanes along k, merged by shuffles.
+template <typename T>
+__global__ __launch_bounds__(256) void stats_cols_kernel(const T* __restrict__ x, int64_t rows,
+                                                         int64_t K, int64_t s_row, int64_t s_k,
+                                                         int64_t kchunk, int* emax, long long* esum,
+                                                         int* ecnt) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = blockIdx.x * (int64_t)4 + (threadIdx.x >> 6);
+    if (i >= rows) return;
+    const int64_t p0 = blockIdx.y * kchunk;
+    const int64_t p1 = (p0 + kchunk < K) ? p0 + kchunk : K;
+    const T* xr = x + i * s_row;
+    ExpStat a;
+    int64_t p = p0 + lane;
+    for (; p + 448 < p1; p += 512) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = xr[(p + 64 * u) * s_k];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) stat_one(v[u], a);
+    }
+    for (; p < p1; p += 64) stat_one(xr[p * s_k], a);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        ExpStat b;
+        b.m = __shfl_xor(a.m, off);
+        b.cnt = __shfl_xor(a.cnt, off);
+        b.sum = __shfl_xor(a.sum, off);
+        stat_merge(a, b);
+    }
+    if (lane == 0) stat_commit(a, i, emax, esum, ecnt);
 }
 
 // ---- slicing ---------------------------------------------------------------
 // Digits of X_eff in the layout the GEMM reads:
 //   out[((kc * rows_pad + i) * S + t) * 64 + (p % 64)],  kc = p / 64,
 // so a tile of 64 rows at one k chunk is one contiguous block of 64*S*64
-// bytes. Rows >= rows and p >= K are zero. Each thread makes 16 consecutive
-// p of one row; four neighbouring threads cover one row's 64-wide chunk.
+// bytes. Rows >= rows and p >= K are zero. A workgroup makes one such block
+// at a time: each thread slices 16 consecutive p of one row (threads along
+// rows when rows are adjacent in memory, along k otherwise, for coalesced
+// reads), the digits are gathered in LDS, and the block is written out
+// contiguously.
+constexpr int SLICE_LDS = 64 * SPLIT_MAX_SLICES * KC;
+
 template <typename T>
 __global__ __launch_bounds__(256) void slice_kernel(const T* __restrict__ x, int64_t rows,
                                                     int64_t rows_pad, int64_t K, int64_t nkc,
-                                                    int64_t s_row, int64_t s_k,
+                                                    int64_t s_row, int64_t s_k, int row_contig,
                                                     const int* __restrict__ emax, int S,
                                                     signed char* __restrict__ out) {
-    const int64_t total = nkc * rows_pad * 4;
-    for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < total;
-         w += (int64_t)gridDim.x * blockDim.x) {
-        const int sub = (int)(w & 3);
-        const int64_t rest = w >> 2;
-        const int64_t i = rest % rows_pad, kc = rest / rows_pad;
-        double r[16];
+    __shared__ __attribute__((aligned(16))) signed char blk[SLICE_LDS];
+    const int tid = threadIdx.x;
+    const int r = row_contig ? (tid & 63) : (tid >> 2);
+    const int sub = row_contig ? (tid >> 6) : (tid & 3);
+    const int64_t rblocks = rows_pad / 64, tiles = rblocks * nkc;
+    const int chunks = 64 * S * KC / 16;                 // 16-byte chunks per block
+    for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const int64_t rb = tile % rblocks, kc = tile / rblocks;
+        const int64_t i = rb * 64 + r;
+        double v[16];
         if (i < rows) {
             const int em = emax[i];
             const int e = (em == INT_MIN) ? 0 : em;
+            const int64_t pb = kc * KC + sub * 16;
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
-                const int64_t p = kc * KC + sub * 16 + q;
-                const double v = (p < K) ? (double)x[i * s_row + p * s_k] : 0.0;
-                r[q] = ldexp(v, -e);            // |r| < 1, exact
+                const int64_t p = pb + q;
+                v[q] = (p < K) ? ldexp((double)x[i * s_row + p * s_k], -e) : 0.0;  // exact
             }
         } else {
 #pragma unroll
-            for (int q = 0; q < 16; ++q) r[q] = 0.0;
+            for (int q = 0; q < 16; ++q) v[q] = 0.0;
         }
-        signed char* dst = out + ((kc * rows_pad + i) * S) * KC + sub * 16;
         for (int t = 0; t < S; ++t) {
-            union { signed char b[16]; i4_t v; } dg;
+            union { signed char b[16]; i4_t w; } dg;
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
-                const double y = r[q] * 128.0;    // exact
+                const double y = v[q] * 128.0;    // exact
                 const double tr = trunc(y);       // |tr| <= 127
                 dg.b[q] = (signed char)(int)tr;
-                r[q] = y - tr;                    // exact, |r| < 1
+                v[q] = y - tr;                    // exact, |v| < 1
             }
-            *reinterpret_cast<i4_t*>(dst + t * KC) = dg.v;
+            *reinterpret_cast<i4_t*>(blk + ((r * S + t) * KC + sub * 16)) = dg.w;
         }
+        __syncthreads();
+        i4_t* dst = reinterpret_cast<i4_t*>(out + (kc * rows_pad + rb * 64) * S * KC);
+        const i4_t* src = reinterpret_cast<const i4_t*>(blk);
+        for (int c = tid; c < chunks; c += 256) dst[c] = src[c];
+        __syncthreads();
     }
 }
 
@@ -154,17 +256,34 @@ __global__ __launch_bounds__(256) void slice_kernel(const T* __restrict__ x, int
 // MFMA blocks of 16 x 16). Per 64-wide k chunk a wave holds the fragments of
 // all S Y-slices in registers and streams the X-slices, so each slice
 // fragment read from LDS feeds up to S products.
+//
+// Staging: the k chunk of a 64-row tile is one contiguous block of 64*S*64
+// bytes in global memory. It is copied straight into LDS with
+// global_load_lds_dwordx4: lane l of an instruction fills LDS bytes
+// [base + 16 l, base + 16 l + 16), so the lanes choose the source chunks. The
+// LDS row (S*64 bytes, no padding) holds its 16-byte chunks in swizzled
+// order (swz), so a fragment read, 16 rows at one chunk, touches 16
+// different bank groups. There are no staging registers, and one buffer of
+// 2*64*S*64 bytes lets two workgroups share a CU.
 template <int S>
 struct SliceTile {
     static constexpr int NT = 256;
-    static constexpr int ROWB = KC * S;                 // bytes of one row's chunk
-    static constexpr int PITCH = ROWB + 16;             // LDS row pitch
-    static constexpr int OPB = 64 * PITCH;              // one operand's LDS tile
-    static constexpr int STAGE = 2 * OPB;
-    static constexpr int NBUF = (2 * STAGE <= 160 * 1024) ? 2 : 1;
-    static constexpr int NLD = 64 * ROWB / 16 / NT;     // 16-byte loads per thread per operand
-    static_assert(NLD == S, "each thread stages S chunks per operand");
+    static constexpr int CPR = 4 * S;                   // 16-byte chunks per row
+    static constexpr int OPB = 64 * CPR * 16;           // one operand's LDS tile
+    static constexpr int LDS = 2 * OPB;
+    static constexpr int ROUNDS = 64 * CPR / NT;        // load instructions per wave per operand
+    static_assert(ROUNDS == S, "each wave issues S loads per operand");
 };
+
+// Position of logical chunk c of row r inside the LDS row (an involution).
+// Chunks are permuted by XOR within aligned groups of 16 (of 4 in a
+// trailing partial group).
+template <int S>
+__device__ __forceinline__ int swz(int r, int c) {
+    constexpr int CPR = 4 * S;
+    return ((c | 15) < CPR) ? ((c & ~15) | ((c & 15) ^ (r & 15)))
+                            : ((c & ~3) | ((c & 3) ^ (r & 3)));
+}
 
 struct SliceGemmArgs {
     const signed char* xs;   // slices of X, rows padded to x_pad
@@ -178,10 +297,13 @@ struct SliceGemmArgs {
     double alpha, beta;
 };
 
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
 template <typename T, int S>
-__global__ __launch_bounds__(256) void slice_gemm_kernel(SliceGemmArgs g) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 8 ? 2 : 1, S <= 8 ? 2 : 1)))
+void slice_gemm_kernel(SliceGemmArgs g) {
     using TL = SliceTile<S>;
-    __shared__ __attribute__((aligned(16))) signed char smem[TL::NBUF * TL::STAGE];
+    __shared__ __attribute__((aligned(16))) signed char smem[TL::LDS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wx = (wave & 1) * 32, wy = (wave >> 1) * 32;
     const int64_t x0 = (int64_t)blockIdx.x * 64, y0 = (int64_t)blockIdx.y * 64;
@@ -194,51 +316,50 @@ __global__ __launch_bounds__(256) void slice_gemm_kernel(SliceGemmArgs g) {
 #pragma unroll
             for (int j = 0; j < 2; ++j) acc[c][i][j] = i4_t{0, 0, 0, 0};
 
-    i4_t rx[S], ry[S];
-    auto gload = [&](int64_t kc) {
-        const i4_t* gx = reinterpret_cast<const i4_t*>(g.xs + ((kc * g.x_pad + x0) * S) * KC);
-        const i4_t* gy = reinterpret_cast<const i4_t*>(g.ys + ((kc * g.y_pad + y0) * S) * KC);
+    // Source chunk of every (round, lane): LDS position P = (q*4 + wave)*64 +
+    // lane holds logical chunk swz(r, P % CPR) of row r = P / CPR.
+    int src[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        const int P = (q * 4 + wave) * 64 + lane;
+        const int r = P / TL::CPR, cp = P % TL::CPR;
+        src[q] = (r * TL::CPR + swz<S>(r, cp)) * 16;
+    }
+    auto stage = [&](int64_t kc) {
+        const signed char* gx = g.xs + ((kc * g.x_pad + x0) * S) * KC;
+        const signed char* gy = g.ys + ((kc * g.y_pad + y0) * S) * KC;
 #pragma unroll
         for (int q = 0; q < S; ++q) {
-            rx[q] = gx[q * TL::NT + tid];
-            ry[q] = gy[q * TL::NT + tid];
-        }
-    };
-    auto sstore = [&](signed char* st) {
-#pragma unroll
-        for (int q = 0; q < S; ++q) {
-            const int v = q * TL::NT + tid;
-            const int row = v / (TL::ROWB / 16), off = (v % (TL::ROWB / 16)) * 16;
-            *reinterpret_cast<i4_t*>(st + row * TL::PITCH + off) = rx[q];
-            *reinterpret_cast<i4_t*>(st + TL::OPB + row * TL::PITCH + off) = ry[q];
+            signed char* dx = smem + (q * 4 + wave) * 64 * 16;
+            __builtin_amdgcn_global_load_lds((const void*)(gx + src[q]), (lds_ptr_t)dx, 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void*)(gy + src[q]), (lds_ptr_t)(dx + TL::OPB),
+                                             16, 0, 0);
         }
     };
 
-    const int lr = lane & 15, lk = (lane >> 4) * 16;
-    gload(g.kc0);
-    sstore(smem);
-    __syncthreads();
+    const int lr = lane & 15, kk = lane >> 4;
+    stage(g.kc0);
     for (int64_t kc = g.kc0; kc < g.kc1; ++kc) {
-        const int cur = (TL::NBUF == 2) ? (int)((kc - g.kc0) & 1) : 0;
-        const signed char* sx = smem + cur * TL::STAGE;
-        const signed char* sy = sx + TL::OPB;
-        const bool more = kc + 1 < g.kc1;
-        if (more) gload(kc + 1);
-
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        const signed char* sx = smem;
+        const signed char* sy = smem + TL::OPB;
         i4_t yf[S][2];
 #pragma unroll
         for (int u = 0; u < S; ++u)
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
-                yf[u][j] = *reinterpret_cast<const i4_t*>(sy + (wy + 16 * j + lr) * TL::PITCH +
-                                                           u * KC + lk);
+            for (int j = 0; j < 2; ++j) {
+                const int r = wy + 16 * j + lr;
+                yf[u][j] = *reinterpret_cast<const i4_t*>(sy + (r * TL::CPR + swz<S>(r, 4 * u + kk)) * 16);
+            }
 #pragma unroll
         for (int t = 0; t < S; ++t) {
             i4_t xf[2];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
-                xf[i] = *reinterpret_cast<const i4_t*>(sx + (wx + 16 * i + lr) * TL::PITCH +
-                                                        t * KC + lk);
+            for (int i = 0; i < 2; ++i) {
+                const int r = wx + 16 * i + lr;
+                xf[i] = *reinterpret_cast<const i4_t*>(sx + (r * TL::CPR + swz<S>(r, 4 * t + kk)) * 16);
+            }
 #pragma unroll
             for (int u = 0; u + t < S; ++u)
 #pragma unroll
@@ -248,9 +369,8 @@ __global__ __launch_bounds__(256) void slice_gemm_kernel(SliceGemmArgs g) {
                         acc[t + u][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
                             xf[i], yf[u][j], acc[t + u][i][j], 0, 0, 0);
         }
-        if (TL::NBUF == 1) __syncthreads();
-        if (more) sstore(smem + ((TL::NBUF == 2) ? (cur ^ 1) : 0) * TL::STAGE);
-        __syncthreads();
+        __syncthreads();                 // every wave is done with this chunk
+        if (kc + 1 < g.kc1) stage(kc + 1);
     }
 
     // Epilogue: class c (weight d = c + 2) carries 2^-7(c+2).
@@ -307,29 +427,41 @@ static void launch_stats(Dt dt, View v, int64_t rows, int64_t K, int64_t s_row, 
                          int* emax, long long* esum, int* ecnt, hipStream_t st) {
     hipLaunchKernelGGL(stats_init_kernel, dim3(grid1(rows)), dim3(256), 0, st, emax, esum, ecnt,
                        rows);
-    const bool row_contig = (s_row == 1);
-    const int64_t kchunk = 512;
-    const dim3 grid(row_contig ? (unsigned)((rows + 255) / 256) : (unsigned)((rows + 3) / 4),
-                    (unsigned)((K + kchunk - 1) / kchunk));
-    if (dt == Dt::F64)
-        hipLaunchKernelGGL(stats_kernel<double>, grid, dim3(256), 0, st, (const double*)v.ptr,
-                           rows, K, s_row, s_k, (int)row_contig, kchunk, emax, esum, ecnt);
-    else
-        hipLaunchKernelGGL(stats_kernel<float>, grid, dim3(256), 0, st, (const float*)v.ptr,
-                           rows, K, s_row, s_k, (int)row_contig, kchunk, emax, esum, ecnt);
+    if (s_row == 1) {
+        const int64_t kchunk = 256;
+        const dim3 grid((unsigned)((rows + 63) / 64), (unsigned)((K + kchunk - 1) / kchunk));
+        if (dt == Dt::F64)
+            hipLaunchKernelGGL(stats_rows_kernel<double>, grid, dim3(256), 0, st,
+                               (const double*)v.ptr, rows, K, s_k, kchunk, emax, esum, ecnt);
+        else
+            hipLaunchKernelGGL(stats_rows_kernel<float>, grid, dim3(256), 0, st,
+                               (const float*)v.ptr, rows, K, s_k, kchunk, emax, esum, ecnt);
+    } else {
+        const int64_t kchunk = 2048;
+        const dim3 grid((unsigned)((rows + 3) / 4), (unsigned)((K + kchunk - 1) / kchunk));
+        if (dt == Dt::F64)
+            hipLaunchKernelGGL(stats_cols_kernel<double>, grid, dim3(256), 0, st,
+                               (const double*)v.ptr, rows, K, s_row, s_k, kchunk, emax, esum, ecnt);
+        else
+            hipLaunchKernelGGL(stats_cols_kernel<float>, grid, dim3(256), 0, st,
+                               (const float*)v.ptr, rows, K, s_row, s_k, kchunk, emax, esum, ecnt);
+    }
     HIP_CHECK(hipGetLastError());
 }
 
 static void launch_slice(Dt dt, View v, int64_t rows, int64_t rows_pad, int64_t K, int64_t nkc,
                          int64_t s_row, int64_t s_k, const int* emax, int S, signed char* out,
                          hipStream_t st) {
-    const unsigned grid = grid1(nkc * rows_pad * 4);
+    const int64_t tiles = rows_pad / 64 * nkc;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(tiles, 8192));
+    const int row_contig = (s_row == 1) ? 1 : 0;
     if (dt == Dt::F64)
         hipLaunchKernelGGL(slice_kernel<double>, dim3(grid), dim3(256), 0, st,
-                           (const double*)v.ptr, rows, rows_pad, K, nkc, s_row, s_k, emax, S, out);
+                           (const double*)v.ptr, rows, rows_pad, K, nkc, s_row, s_k, row_contig,
+                           emax, S, out);
     else
         hipLaunchKernelGGL(slice_kernel<float>, dim3(grid), dim3(256), 0, st, (const float*)v.ptr,
-                           rows, rows_pad, K, nkc, s_row, s_k, emax, S, out);
+                           rows, rows_pad, K, nkc, s_row, s_k, row_contig, emax, S, out);
     HIP_CHECK(hipGetLastError());
 }
 

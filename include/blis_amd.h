@@ -90,6 +90,55 @@ int bli_amd_gemm_split_slices( int is_double, double gap_bits );
    native path (the break-even point against native MFMA). */
 int bli_amd_gemm_split_max_slices( int is_double );
 
+/* ---- configuration space, autotuner and tile selector -------------------
+   A GEMM configuration is a tile of the gfx950 tile table combined with a
+   tile schedule (raster group height, XCD remapping). All configurations
+   compute bit-identical results; they differ in speed only.
+   Configuration c: tile = c / 3, schedule = c % 3. */
+int  bli_amd_gemm_config_count( int is_double );
+int  bli_amd_gemm_config_describe( int is_double, int cfg, int* tile, int* group_m,
+                                   int* xcd_remap, int* bm, int* bn, int* bk );
+/* Time every configuration on one column-major shape (library-owned random
+   operands with the given leading dimensions); ms receives
+   bli_amd_gemm_config_count() milliseconds. Returns 0, or -1 for invalid
+   arguments. */
+int  bli_amd_gemm_autotune( int is_double, trans_t transa, trans_t transb,
+                            dim_t m, dim_t n, dim_t k, inc_t lda, inc_t ldb, inc_t ldc,
+                            float* ms );
+/* The learned selector's 16 input features for a column-major shape. */
+void bli_amd_gemm_features( int is_double, trans_t transa, trans_t transb,
+                            dim_t m, dim_t n, dim_t k, inc_t lda, inc_t ldb, inc_t ldc,
+                            float* out );
+/* How GEMMs without an explicit configuration pick one:
+   0 = built-in size heuristic (default), 1 = learned (the MLP net, or the
+   library's shipped weights when net is NULL), 2 = exhaustive autotuning on
+   first use of a shape. Choices are cached per shape. The environment
+   variable BLIS_AMD_SELECTOR={heuristic,mlp,exhaustive} sets the initial
+   mode (BLIS_AMD_SELECTOR_WEIGHTS names a weight file). */
+void bli_amd_gemm_set_selector( int mode, void* net );
+int  bli_amd_gemm_get_selector( void );
+int  bli_amd_gemm_selected_config( int is_double, trans_t transa, trans_t transb,
+                                   dim_t m, dim_t n, dim_t k, inc_t lda, inc_t ldb, inc_t ldc );
+
+/* ---- the selector network (FP32 MLP 16 -> 64 -> 64 -> configurations) ----
+   Trained and evaluated by HIP kernels. Targets are log2(t_c / t_best) per
+   configuration; the prediction is the argmin. Parameters are laid out as
+   W1[64][16] b1[64] W2[64][64] b2[64] W3[C][64] b3[C]; mu_sd holds the
+   feature normalisation mu[16] then sd[16]. Feature/target arrays are host
+   arrays, row-major. */
+int    bli_amd_mlp_param_count( void );
+void*  bli_amd_mlp_create( unsigned seed );
+void   bli_amd_mlp_destroy( void* net );
+void   bli_amd_mlp_set_params( void* net, const float* params, const float* mu_sd );
+void   bli_amd_mlp_get_params( void* net, float* params, float* mu_sd );
+double bli_amd_mlp_loss_and_grad( void* net, const float* feats, const float* targets, int n,
+                                  float* grad );
+double bli_amd_mlp_train( void* net, const float* feats, const float* targets, int n,
+                          int epochs, int batch, float lr, unsigned seed );
+void   bli_amd_mlp_predict( void* net, const float* feats, int n, float* scores, int* best );
+int    bli_amd_mlp_save( void* net, const char* path );
+void*  bli_amd_mlp_load( const char* path );
+
 #ifdef __cplusplus
 }
 #endif
