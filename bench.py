@@ -214,6 +214,14 @@ def extras(bench: Bench):
 
     # Split-precision DGEMM/SGEMM 8192^3 vs native at matched error.
     out.update(split_bench(bench, log))
+
+    # Learned tile selector: held-out regret from the shipped evaluation.
+    sel = ROOT / "blis_amd" / "data" / "selector_gfx950.json"
+    if sel.exists():
+        j = json.loads(sel.read_text())
+        out["selector_holdout"] = {"learned": j["learned_holdout"],
+                                   "heuristic": j["heuristic_holdout"],
+                                   "shapes": j["shapes"], "holdout": j["holdout"]}
     return out
 
 
@@ -368,6 +376,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-split", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="one 1 GiB device copy, then two headline DGEMM calls (for "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -378,6 +389,19 @@ def main():
     bench = Bench()
     if args.only_split:
         print(json.dumps(split_bench(bench, log)), flush=True)
+        return
+    if args.traffic_probe:
+        # Calibration: a plain device copy of exactly 1 GiB read + 1 GiB written.
+        src = bench.uniform(1 << 27, torch.float64)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        step = bench.gemm_fn("d", 0, 0, 8192, 8192, 8192)
+        step()
+        step()
+        torch.cuda.synchronize()
+        log("traffic probe: 1 GiB copy, 2 x DGEMM NN 8192^3 done")
         return
     m = n = k = 8192
     log(f"headline: DGEMM NN {m}x{n}x{k}, alpha=1, beta=0.5, warmup {args.warmup}, steps {args.steps}")

@@ -189,3 +189,50 @@ def test_selector_modes_bit_identical_to_heuristic(blis, L, mode, tmp_path):
         L.bli_amd_gemm_set_selector(0, None)
         if net:
             L.bli_amd_mlp_destroy(net)
+
+
+def test_shipped_weights_predict_like_the_model(L):
+    """The HIP inference kernels and the CPU model agree on the held-out
+    shapes of the tuning database (FP32 vs FP64 arithmetic may flip a pick
+    only between configurations with near-equal scores)."""
+    from pathlib import Path
+    say("shipped selector predictions")
+    data = Path(__file__).resolve().parent.parent / "blis_amd" / "data"
+    db = np.load(data / "gemm_tuning_gfx950.npz")
+    feats = np.ascontiguousarray(db["feats"][20000:], dtype=np.float32)
+    ms = db["ms"][20000:].astype(np.float64)
+    net = L.bli_amd_mlp_load(str(data / "selector_gfx950.bin").encode())
+    assert net
+    n = len(feats)
+    best = np.zeros(n, dtype=np.int32)
+    L.bli_amd_mlp_predict(net, fp(feats), n, None, best.ctypes.data_as(C.POINTER(C.c_int)))
+    L.bli_amd_mlp_destroy(net)
+    p, mu, sd = M.load_weights(data / "selector_gfx950.bin")
+    ref = M.predict(p, feats.astype(np.float64), mu, sd)
+    assert np.mean(best == ref) >= 0.995
+    rows = np.arange(n)
+    tot_gpu = ms[rows, best].sum() / ms.min(axis=1).sum() - 1
+    tot_ref = ms[rows, ref].sum() / ms.min(axis=1).sum() - 1
+    assert abs(tot_gpu - tot_ref) <= 0.002 and tot_gpu <= 0.05
+
+
+def test_learned_mode_with_shipped_weights(blis, L):
+    say("selector mode 1 with shipped weights")
+    L.bli_amd_gemm_set_selector.argtypes = [C.c_int, C.c_void_p]
+    L.bli_amd_gemm_selected_config.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_int64] * 6
+    m, n, k = 1000, 333, 517
+    a = Mat(m, k, seed=15).to_device()
+    b = Mat(k, n, seed=16).to_device()
+    c0 = Mat(m, n, seed=17).to_device()
+    _gemm(blis, "d", m, n, k, a, b, c0)
+    ref = c0.device_view().copy()
+    try:
+        L.bli_amd_gemm_set_selector(1, None)
+        assert L.bli_amd_gemm_get_selector() == 1
+        cfg = L.bli_amd_gemm_selected_config(1, 0, 0, m, n, k, m, k, m)
+        assert 0 <= cfg < 42
+        c1 = Mat(m, n, seed=17).to_device()
+        _gemm(blis, "d", m, n, k, a, b, c1)
+        assert np.array_equal(c1.device_view(), ref)
+    finally:
+        L.bli_amd_gemm_set_selector(0, None)
