@@ -297,3 +297,29 @@ def test_checkers_reject_planted_mistakes(oracle):
     oracle.trsm(0, 0xC0, "n", 0, 1.0, np.asfortranarray(t), x)
     assert residual_trsm(t, x, bb, 1.0, 0) < 32 * u
     assert residual_trsm(t, x * 32, bb, 1.0, 0) > 1e-3
+
+
+def test_double_double_gemm_is_correctly_rounded(oracle):
+    """oracle_dgemm_dd_rows (the reference product for split precision)
+    against exact rational arithmetic: every entry of alpha*A*B + beta*C
+    must be the exact value rounded to the nearest double (up to one ulp on
+    rare near-ties), also with catastrophic cancellation."""
+    from fractions import Fraction
+    rng = np.random.default_rng(12)
+    m, n, k = 5, 4, 60
+    a = np.asfortranarray(rng.uniform(-1, 1, (m, k)) * np.exp2(rng.integers(-20, 20, (m, k))))
+    b = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+    b[:, 0] = 0.0
+    b[0, 0], b[1, 0] = 1.0, 1.0
+    a[:, 1] = -a[:, 0]          # column 0 of C cancels exactly to zero
+    c0 = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    alpha, beta = 0.75, -1.5
+    got = np.asfortranarray(c0.copy())
+    oracle.gemm_dd_rows("n", "n", alpha, a, b, beta, got, 0, m)
+    for i in range(m):
+        for j in range(n):
+            exact = Fraction(alpha) * sum(Fraction(a[i, p]) * Fraction(b[p, j]) for p in range(k)) \
+                + Fraction(beta) * Fraction(c0[i, j])
+            want = float(exact)          # correctly rounded
+            assert abs(got[i, j] - want) <= np.spacing(abs(want)), (i, j, got[i, j], want)
+    assert np.all(got[:, 0] == beta * c0[:, 0])
