@@ -349,8 +349,14 @@ def batch_bench(bench, ch, key, log, count=10000, seed=42):
     gs = I(*([1] * count))
     cargs = [C.cast(x, C.c_void_p) for x in args] + [count, C.cast(gs, C.c_void_p)]
     plan = getattr(L, f"bli_amd_{ch}gemm_batch_plan_create")(*cargs)
+    modes = {}
+    for mode in range(5):
+        L.bli_amd_gemm_batch_plan_set_mode(plan, mode)
+        modes[mode] = round(bench.kernel_ms(lambda: L.bli_amd_gemm_batch_plan_execute(plan, 1), 10), 4)
+    tuned = L.bli_amd_gemm_batch_plan_autotune(plan, 5)
     ms = bench.kernel_ms(lambda: L.bli_amd_gemm_batch_plan_execute(plan, 1), 20)
     L.bli_amd_gemm_batch_plan_destroy(plan)
+    log(f"  {ch}gemm_batch modes (ms): {modes}; autotuned mode {tuned}")
     m, n, k = dims[:, 0].astype(float), dims[:, 1].astype(float), dims[:, 2].astype(float)
     flops = 2 * m * n * k
     byts = es * (m * k + k * n + 2 * m * n)
@@ -360,7 +366,8 @@ def batch_bench(bench, ch, key, log, count=10000, seed=42):
     res = {f"{ch}gemm_batch_10000": {"ms": round(ms, 4), "TFLOP/s": round(tf, 3),
                                      "roofline_ms": round(roof_ms, 4),
                                      "frac": round(roof_ms / ms, 4),
-                                     "bound": "per-problem max(mfma, hbm)"}}
+                                     "bound": "per-problem max(mfma, hbm)",
+                                     "mode_ms": modes, "autotuned_mode": tuned}}
     log(f"  {ch}gemm_batch_10000{'':17s} {ms:9.4f} ms  {tf:9.3f} TFLOP/s  "
         f"({100 * roof_ms / ms:5.1f}% of per-problem roofline {roof_ms:.3f} ms)")
     del A, B, Cm

@@ -109,6 +109,9 @@ def _sig(lib):
     s("bli_amd_gemm_batch_plan_execute", None, P, C.c_int)
     s("bli_amd_gemm_batch_plan_destroy", None, P)
     s("bli_amd_gemm_batch_plan_work_items", C.c_int, P)
+    s("bli_amd_gemm_batch_plan_autotune", C.c_int, P, C.c_int)
+    s("bli_amd_gemm_batch_plan_set_mode", None, P, C.c_int)
+    s("bli_amd_gemm_batch_plan_get_mode", C.c_int, P)
     s("bli_amd_set_stream", None, P, C.c_int)
     s("bli_amd_get_stream", P)
     s("bli_amd_synchronize", None)

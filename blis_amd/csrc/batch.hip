@@ -39,8 +39,8 @@ struct BatchDesc {
     int32_t     m, n, k;
     int32_t     flags;
     double      alpha, beta;
-    int32_t     tiles_m;
-    int32_t     pad;
+    int32_t     tiles_m;     // tiles along m at the problem's own tile size
+    int32_t     tiles_m32;   // tiles along m at 32 (wave-tile mode)
 };
 
 struct WorkItem {
@@ -58,25 +58,40 @@ template <> struct BatchTiles<float> {
     using Small = TileCfg<float, 32, 32, 32, 16, 16>;
 };
 
-template <typename T, class TL>
-__device__ inline void run_tile(const TileProblem<T>& p, int32_t flags, int64_t tm, int64_t tn,
-                                T* smem) {
-    const bool a_k = flags & F_A_K, b_k = flags & F_B_K;
-    if (!a_k && !b_k) gemm_tile<T, TL, false, false>(p, tm, tn, smem);
-    else if (!a_k && b_k) gemm_tile<T, TL, false, true>(p, tm, tn, smem);
-    else if (a_k && !b_k) gemm_tile<T, TL, true, false>(p, tm, tn, smem);
-    else gemm_tile<T, TL, true, true>(p, tm, tn, smem);
-}
+// One persistent kernel per (tile shape, A layout, B layout) variant: each
+// instantiation holds only its own tile code, so its register count (and
+// with it the number of resident workgroups) is that of the plain GEMM
+// kernel with the same tile, and its code fits the instruction cache.
+// Work items of one variant, split into one list per XCD: list x is
+// items[off[x] .. off[x+1]). Workgroups are dispatched round-robin over the
+// 8 XCDs, so workgroup b (on XCD b % 8) walks list b % 8; every tile of a
+// problem then runs on one XCD and its operands are fetched into one L2.
+constexpr int NXCD = 8;
+struct XcdSegs {
+    int off[NXCD + 1];
+};
 
-template <typename T>
-__global__ __launch_bounds__(256) void gemm_batch_kernel(const BatchDesc* __restrict__ descs,
-                                                         const WorkItem* __restrict__ items,
-                                                         int nitems) {
-    using Big = typename BatchTiles<T>::Big;
-    using Small = typename BatchTiles<T>::Small;
-    constexpr int SM = 2 * (Big::STAGE > Small::STAGE ? Big::STAGE : Small::STAGE);
-    __shared__ T smem[SM];
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+// Kernel modes (a schedule choice the batch autotuner times; every mode
+// computes bit-identical results):
+//   0: one slab in flight, guarded element loads on edge tiles;
+//   1: one slab in flight, precomputed edge addressing (EdgeState);
+//   2: two slabs in flight, precomputed edge addressing;
+//   3: two slabs in flight, guarded element loads;
+//   4: wave tiles (gemm_batch_wave_kernel).
+constexpr int NMODE = 5;
+constexpr int WAVE_MODE = 4;
+constexpr int BATCH_DEFAULT_MODE = 0;
+constexpr int mode_pf(int mode) { return mode >= 2 ? 2 : 1; }
+constexpr bool mode_edge(int mode) { return mode == 1 || mode == 2; }
+
+template <typename T, class TL, bool A_K, bool B_K, int MODE>
+__global__ __launch_bounds__(TL::NT) void gemm_batch_kernel(const BatchDesc* __restrict__ descs,
+                                                            const WorkItem* __restrict__ items,
+                                                            XcdSegs seg) {
+    __shared__ T smem[2 * TL::STAGE];
+    const int x = blockIdx.x % NXCD;
+    const int slot = blockIdx.x / NXCD, nslot = gridDim.x / NXCD;   // gridDim.x % NXCD == 0
+    for (int it = seg.off[x] + slot; it < seg.off[x + 1]; it += nslot) {
         const WorkItem w = items[it];
         const BatchDesc& d = descs[w.prob];
         TileProblem<T> p;
@@ -88,9 +103,167 @@ __global__ __launch_bounds__(256) void gemm_batch_kernel(const BatchDesc* __rest
         p.vec_b = (d.flags & F_VEC_B) != 0;
         p.vec_c = (d.flags & F_VEC_C) != 0;
         const int64_t tm = w.tile % d.tiles_m, tn = w.tile / d.tiles_m;
-        if (d.flags & F_SMALL) run_tile<T, Small>(p, d.flags, tm, tn, smem);
-        else run_tile<T, Big>(p, d.flags, tm, tn, smem);
+        gemm_tile<T, TL, A_K, B_K, mode_pf(MODE), mode_edge(MODE)>(p, tm, tn, smem);
         __syncthreads();
+    }
+}
+
+// Wave-tile kernel (mode 4): one wave computes one 32 x 32 tile of C as 2 x 2
+// MFMA blocks of 16 x 16, loading its MFMA operand fragments straight from
+// global memory (no LDS, no barriers), one k-step ahead. For the small
+// problems of a batch this wastes less work on padding than 64 x 64 tiles and
+// needs few registers, so many waves per SIMD hide the memory latency.
+// Lane l supplies A(row 16i + l%16, k 4s + l/16) and B(k 4s + l/16,
+// column 16j + l%16) to k-step s: the same MFMA sequence per output element
+// as the LDS-staged tiles, hence bit-identical results.
+template <typename T, bool A_K, bool B_K>
+__global__ __launch_bounds__(256) void gemm_batch_wave_kernel(const BatchDesc* __restrict__ descs,
+                                                              const WorkItem* __restrict__ items,
+                                                              XcdSegs seg) {
+    using acc_t = typename Mma<T>::acc_t;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x = blockIdx.x % NXCD;
+    const int slot = (blockIdx.x / NXCD) * 4 + wave, nslot = (gridDim.x / NXCD) * 4;
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int it = seg.off[x] + slot; it < seg.off[x + 1]; it += nslot) {
+        const WorkItem w = items[it];
+        const BatchDesc& d = descs[w.prob];
+        const T* __restrict__ a = (const T*)d.a;
+        const T* __restrict__ b = (const T*)d.b;
+        const int64_t m = d.m, n = d.n, K = d.k, lda = d.lda, ldb = d.ldb;
+        const int64_t m0 = (int64_t)(w.tile % d.tiles_m32) * 32, n0 = (int64_t)(w.tile / d.tiles_m32) * 32;
+        // Rows / columns this lane feeds, clamped into the matrix.
+        int64_t ar[2], bc[2];
+        bool aok[2], bok[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int64_t r = m0 + 16 * i + lr, c = n0 + 16 * i + lr;
+            aok[i] = r < m;
+            bok[i] = c < n;
+            ar[i] = (aok[i] ? r : m - 1) * (A_K ? lda : 1);
+            bc[i] = (bok[i] ? c : n - 1) * (B_K ? ldb : 1);
+        }
+        acc_t acc[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{};
+        const int64_t nks = (K + 3) / 4;
+        auto frag = [&](int64_t s, T (&fa)[2], T (&fb)[2]) {
+            const int64_t kk = 4 * s + lk;
+            const bool kin = kk < K;
+            const int64_t kc = kin ? kk : K - 1;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const T va = a[ar[i] + kc * (A_K ? 1 : lda)];
+                const T vb = b[bc[i] + kc * (B_K ? 1 : ldb)];
+                fa[i] = (kin && aok[i]) ? va : T(0);
+                fb[i] = (kin && bok[i]) ? vb : T(0);
+            }
+        };
+        if (nks > 0) {
+            T fa[2], fb[2], ga[2], gb[2];
+            frag(0, fa, fb);
+            for (int64_t s = 0; s < nks; s += 2) {
+                if (s + 1 < nks) frag(s + 1, ga, gb);
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) acc[i][j] = Mma<T>::mma(fa[i], fb[j], acc[i][j]);
+                if (s + 1 < nks) {
+                    if (s + 2 < nks) frag(s + 2, fa, fb);
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+#pragma unroll
+                        for (int j = 0; j < 2; ++j) acc[i][j] = Mma<T>::mma(ga[i], gb[j], acc[i][j]);
+                }
+            }
+        }
+        // Store: register r of block (i, j) holds row 16i + acc_row(r, lk),
+        // column 16j + lr.
+        T* c = (T*)d.c;
+        const T alpha = (T)d.alpha, beta = (T)d.beta;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int64_t col = n0 + 16 * j + lr;
+            if (col >= n) continue;
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t row = m0 + 16 * i + Mma<T>::acc_row(r, lk);
+                    if (row >= m) continue;
+                    T* dst = c + row * d.c_si + col * d.c_sj;
+                    *dst = (beta == T(0)) ? alpha * acc[i][j][r]
+                                          : epilogue(alpha, acc[i][j][r], beta, *dst);
+                }
+        }
+    }
+}
+
+// Variant v = 4*small + 2*a_k + b_k.
+constexpr int NVARIANT = 8;
+inline int variant_of(int32_t flags) {
+    return ((flags & F_SMALL) ? 4 : 0) + ((flags & F_A_K) ? 2 : 0) + ((flags & F_B_K) ? 1 : 0);
+}
+
+template <typename T, int MODE>
+inline const void* variant_fn(int v) {
+    using Big = typename BatchTiles<T>::Big;
+    using Small = typename BatchTiles<T>::Small;
+    switch (v) {
+    case 0: return (const void*)&gemm_batch_kernel<T, Big, false, false, MODE>;
+    case 1: return (const void*)&gemm_batch_kernel<T, Big, false, true, MODE>;
+    case 2: return (const void*)&gemm_batch_kernel<T, Big, true, false, MODE>;
+    case 3: return (const void*)&gemm_batch_kernel<T, Big, true, true, MODE>;
+    case 4: return (const void*)&gemm_batch_kernel<T, Small, false, false, MODE>;
+    case 5: return (const void*)&gemm_batch_kernel<T, Small, false, true, MODE>;
+    case 6: return (const void*)&gemm_batch_kernel<T, Small, true, false, MODE>;
+    case 7: return (const void*)&gemm_batch_kernel<T, Small, true, true, MODE>;
+    }
+    abort_with(BLIS_INVALID_UKR_ID, "batch: variant");
+    return nullptr;
+}
+
+template <typename T>
+inline void variant_info(int mode, int v, const void*& fn, int& threads) {
+    threads = (v < 4) ? BatchTiles<T>::Big::NT : BatchTiles<T>::Small::NT;
+    if (mode == WAVE_MODE) {
+        // v = 2*a_k + b_k
+        threads = 256;
+        switch (v) {
+        case 0: fn = (const void*)&gemm_batch_wave_kernel<T, false, false>; return;
+        case 1: fn = (const void*)&gemm_batch_wave_kernel<T, false, true>; return;
+        case 2: fn = (const void*)&gemm_batch_wave_kernel<T, true, false>; return;
+        case 3: fn = (const void*)&gemm_batch_wave_kernel<T, true, true>; return;
+        }
+        abort_with(BLIS_INVALID_UKR_ID, "batch: wave variant");
+    }
+    switch (mode) {
+    case 0: fn = variant_fn<T, 0>(v); return;
+    case 1: fn = variant_fn<T, 1>(v); return;
+    case 2: fn = variant_fn<T, 2>(v); return;
+    case 3: fn = variant_fn<T, 3>(v); return;
+    }
+    abort_with(BLIS_INVALID_UKR_ID, "batch: mode");
+}
+
+// C of every problem to / from a contiguous scratch copy (the autotuner
+// runs the batch several times and restores C afterwards).
+template <typename T>
+__global__ void batch_c_copy_kernel(const BatchDesc* __restrict__ descs, int ndesc,
+                                    const int64_t* __restrict__ offs, T* scratch, int to_scratch) {
+    for (int i = blockIdx.x; i < ndesc; i += gridDim.x) {
+        const BatchDesc& d = descs[i];
+        T* c = (T*)d.c;
+        T* s = scratch + offs[i];
+        const int64_t cnt = (int64_t)d.m * d.n;
+        for (int64_t e = threadIdx.x; e < cnt; e += blockDim.x) {
+            const int64_t r = e % d.m, col = e / d.m;
+            T* cp = c + r * d.c_si + col * d.c_sj;
+            if (to_scratch) s[e] = *cp;
+            else *cp = s[e];
+        }
     }
 }
 
@@ -116,7 +289,14 @@ struct BatchPlan {
     BatchDesc* d_descs = nullptr;
     WorkItem* d_items = nullptr;
     int nitems = 0;
-    int grid = 0;
+    XcdSegs seg[NVARIANT] = {};       // per variant: absolute item offsets per XCD list
+    int grid[NMODE][NVARIANT] = {};   // persistent grid per kernel mode and variant
+    WorkItem* d_witems = nullptr;     // wave-tile mode: 32 x 32 tiles of every problem
+    XcdSegs wseg[4] = {};             // per layout variant (2*a_k + b_k)
+    int mode = 0;                     // kernel mode used by execute
+    int ndesc = 0;
+    int64_t c_elems = 0;              // sum of m*n (autotuner scratch)
+    std::vector<int64_t> c_offs;
     std::vector<BatchProblem> fallback;   // general-stride problems (plain GEMMs)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -126,6 +306,7 @@ struct BatchPlan {
         if (graph) (void)hipGraphDestroy(graph);
         if (d_descs) (void)hipFree(d_descs);
         if (d_items) (void)hipFree(d_items);
+        if (d_witems) (void)hipFree(d_witems);
     }
 };
 
@@ -178,6 +359,7 @@ static BatchPlan* plan_build(Dt dt, const std::vector<BatchProblem>& probs, hipS
         const int tb = small ? 32 : 64;
         d.flags = flags;
         d.tiles_m = (int32_t)((q.m + tb - 1) / tb);
+        d.tiles_m32 = (int32_t)((q.m + 31) / 32);
         const int64_t tiles_n = (q.n + tb - 1) / tb;
         const int32_t pi = (int32_t)descs.size();
         descs.push_back(d);
@@ -186,13 +368,53 @@ static BatchPlan* plan_build(Dt dt, const std::vector<BatchProblem>& probs, hipS
             item_k.push_back(d.k);
         }
     }
-    // Longest-first (by k extent; ties keep submission order).
-    std::vector<int32_t> order(items.size());
+    // Items are grouped by kernel variant, then by XCD list, then
+    // longest-first by k (ties keep submission order). Problems are assigned
+    // to XCD lists greedily, largest first, to the least-loaded list of their
+    // variant (work = tile area x k summed over the problem's tiles).
+    const size_t ni = items.size();
+    std::vector<int32_t> item_v(ni), prob_x(descs.size(), 0);
+    for (size_t i = 0; i < ni; ++i) item_v[i] = variant_of(descs[items[i].prob].flags);
+    {
+        std::vector<double> pwork(descs.size(), 0.0);
+        for (size_t i = 0; i < ni; ++i) {
+            const BatchDesc& d = descs[items[i].prob];
+            const double area = (d.flags & F_SMALL) ? 32.0 * 32.0 : 64.0 * 64.0;
+            pwork[items[i].prob] += area * std::max<int32_t>(d.k, 1);
+        }
+        std::vector<int32_t> porder(descs.size());
+        std::iota(porder.begin(), porder.end(), 0);
+        std::stable_sort(porder.begin(), porder.end(),
+                         [&](int32_t x, int32_t y) { return pwork[x] > pwork[y]; });
+        std::vector<double> load((size_t)NVARIANT * NXCD, 0.0);
+        for (int32_t pi : porder) {
+            if (pwork[pi] == 0.0) continue;
+            const int v = variant_of(descs[pi].flags);
+            int best = 0;
+            for (int x = 1; x < NXCD; ++x)
+                if (load[v * NXCD + x] < load[v * NXCD + best]) best = x;
+            prob_x[pi] = best;
+            load[v * NXCD + best] += pwork[pi];
+        }
+    }
+    std::vector<int32_t> order(ni);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t x, int32_t y) { return item_k[x] > item_k[y]; });
-    std::vector<WorkItem> sorted(items.size());
-    for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        if (item_v[x] != item_v[y]) return item_v[x] < item_v[y];
+        const int32_t px = prob_x[items[x].prob], py = prob_x[items[y].prob];
+        if (px != py) return px < py;
+        return item_k[x] > item_k[y];
+    });
+    std::vector<int> count((size_t)NVARIANT * NXCD, 0);
+    for (size_t i = 0; i < ni; ++i) count[item_v[i] * NXCD + prob_x[items[i].prob]]++;
+    int off = 0;
+    for (int v = 0; v < NVARIANT; ++v)
+        for (int x = 0; x <= NXCD; ++x) {
+            plan->seg[v].off[x] = off;
+            if (x < NXCD) off += count[v * NXCD + x];
+        }
+    std::vector<WorkItem> sorted(ni);
+    for (size_t i = 0; i < ni; ++i) sorted[i] = items[order[i]];
     plan->nitems = (int)sorted.size();
     if (!descs.empty()) {
         HIP_CHECK(hipMalloc(&plan->d_descs, descs.size() * sizeof(BatchDesc)));
@@ -204,22 +426,94 @@ static BatchPlan* plan_build(Dt dt, const std::vector<BatchProblem>& probs, hipS
         HIP_CHECK(hipMemcpyAsync(plan->d_items, sorted.data(), sorted.size() * sizeof(WorkItem),
                                  hipMemcpyHostToDevice, st));
     }
+    // Wave-tile items: every 32 x 32 tile of every problem, grouped by layout
+    // variant, then XCD list (same problem placement as above), then
+    // longest-first by k.
+    {
+        std::vector<WorkItem> witems;
+        std::vector<int32_t> wk, wv, wx;
+        for (size_t pi = 0; pi < descs.size(); ++pi) {
+            const BatchDesc& d = descs[pi];
+            const int64_t t = (int64_t)d.tiles_m32 * ((d.n + 31) / 32);
+            const int lv = variant_of(d.flags) & 3;
+            for (int64_t ti = 0; ti < t; ++ti) {
+                witems.push_back(WorkItem{(int32_t)pi, (int32_t)ti});
+                wk.push_back(d.k);
+                wv.push_back(lv);
+                wx.push_back(prob_x[pi]);
+            }
+        }
+        std::vector<int32_t> word(witems.size());
+        std::iota(word.begin(), word.end(), 0);
+        std::stable_sort(word.begin(), word.end(), [&](int32_t x, int32_t y) {
+            if (wv[x] != wv[y]) return wv[x] < wv[y];
+            if (wx[x] != wx[y]) return wx[x] < wx[y];
+            return wk[x] > wk[y];
+        });
+        std::vector<int> wcnt(4 * NXCD, 0);
+        for (size_t i = 0; i < witems.size(); ++i) wcnt[wv[i] * NXCD + wx[i]]++;
+        int woff = 0;
+        for (int v = 0; v < 4; ++v)
+            for (int x = 0; x <= NXCD; ++x) {
+                plan->wseg[v].off[x] = woff;
+                if (x < NXCD) woff += wcnt[v * NXCD + x];
+            }
+        std::vector<WorkItem> wsorted(witems.size());
+        for (size_t i = 0; i < witems.size(); ++i) wsorted[i] = witems[word[i]];
+        if (!wsorted.empty()) {
+            HIP_CHECK(hipMalloc(&plan->d_witems, wsorted.size() * sizeof(WorkItem)));
+            HIP_CHECK(hipMemcpyAsync(plan->d_witems, wsorted.data(),
+                                     wsorted.size() * sizeof(WorkItem), hipMemcpyHostToDevice, st));
+        }
+    }
     HIP_CHECK(hipStreamSynchronize(st));
-    const int per_cu = 4;
-    plan->grid = std::max(1, std::min(plan->nitems, cu_count() * per_cu));
+    // One resident wave of workgroups per variant and mode (persistent
+    // kernels), a multiple of the XCD count, and no more per XCD than its
+    // longest list.
+    for (int mode = 0; mode < NMODE; ++mode)
+        for (int v = 0; v < NVARIANT; ++v) {
+            const bool wavem = mode == WAVE_MODE;
+            if (wavem && v >= 4) continue;
+            const XcdSegs& sg = wavem ? plan->wseg[v] : plan->seg[v];
+            int longest = 0;
+            for (int x = 0; x < NXCD; ++x) longest = std::max(longest, sg.off[x + 1] - sg.off[x]);
+            if (longest == 0) continue;
+            if (wavem) longest = (longest + 3) / 4;     // four waves per workgroup
+            const void* fn;
+            int threads;
+            if (dt == Dt::F64) variant_info<double>(mode, v, fn, threads);
+            else variant_info<float>(mode, v, fn, threads);
+            int per_cu = 0;
+            HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0));
+            per_cu = std::max(1, per_cu);
+            const int slots = std::max(1, std::min(longest, cu_count() * per_cu / NXCD));
+            plan->grid[mode][v] = slots * NXCD;
+        }
+    plan->mode = BATCH_DEFAULT_MODE;
+    plan->ndesc = (int)descs.size();
+    plan->c_offs.resize(descs.size());
+    for (size_t i = 0; i < descs.size(); ++i) {
+        plan->c_offs[i] = plan->c_elems;
+        plan->c_elems += (int64_t)descs[i].m * descs[i].n;
+    }
     return plan;
 }
 
-static void plan_launch(const BatchPlan* plan, hipStream_t st) {
-    if (plan->nitems > 0) {
-        if (plan->dt == Dt::F64)
-            hipLaunchKernelGGL(gemm_batch_kernel<double>, dim3(plan->grid), dim3(256), 0, st,
-                               plan->d_descs, plan->d_items, plan->nitems);
-        else
-            hipLaunchKernelGGL(gemm_batch_kernel<float>, dim3(plan->grid), dim3(256), 0, st,
-                               plan->d_descs, plan->d_items, plan->nitems);
-        HIP_CHECK(hipGetLastError());
+static void plan_launch(const BatchPlan* plan, hipStream_t st, bool with_fallback = true) {
+    const int mode = plan->mode;
+    for (int v = 0; v < NVARIANT; ++v) {
+        if (plan->grid[mode][v] == 0) continue;
+        const void* fn;
+        int threads;
+        if (plan->dt == Dt::F64) variant_info<double>(mode, v, fn, threads);
+        else variant_info<float>(mode, v, fn, threads);
+        const BatchDesc* descs = plan->d_descs;
+        const WorkItem* items = (mode == WAVE_MODE) ? plan->d_witems : plan->d_items;
+        XcdSegs seg = (mode == WAVE_MODE) ? plan->wseg[v] : plan->seg[v];
+        void* args[] = {(void*)&descs, (void*)&items, (void*)&seg};
+        HIP_CHECK(hipLaunchKernel(fn, dim3(plan->grid[mode][v]), dim3(threads), args, 0, st));
     }
+    if (!with_fallback) return;
     for (const BatchProblem& q : plan->fallback) {
         GemmDesc d;
         d.dt = plan->dt; d.m = q.m; d.n = q.n; d.k = q.k;
@@ -318,6 +612,84 @@ void bli_amd_gemm_batch_plan_execute(void* handle, int use_graph) {
         plan_launch(plan, st);
     }
     sync_if_needed();
+}
+
+// Choose the plan's kernel mode by timing every mode on the plan itself.
+// C is saved before and restored after, so the call leaves the caller's
+// data unchanged. Returns the chosen mode.
+__attribute__((visibility("default")))
+int bli_amd_gemm_batch_plan_autotune(void* handle, int reps) {
+    auto* plan = (BatchPlan*)handle;
+    std::lock_guard<std::mutex> lock(api_mutex());
+    if (plan->nitems == 0) return plan->mode;
+    hipStream_t st = stream();
+    reps = std::max(1, reps);
+    const size_t es = dt_size(plan->dt);
+    void* scratch = workspace((size_t)std::max<int64_t>(plan->c_elems, 1) * es, 11);
+    int64_t* d_offs = (int64_t*)workspace(sizeof(int64_t) * std::max<size_t>(plan->c_offs.size(), 1), 12);
+    HIP_CHECK(hipMemcpyAsync(d_offs, plan->c_offs.data(), sizeof(int64_t) * plan->c_offs.size(),
+                             hipMemcpyHostToDevice, st));
+    auto copy_c = [&](int to_scratch) {
+        const unsigned g = (unsigned)std::max(1, std::min(plan->ndesc, 4096));
+        if (plan->dt == Dt::F64)
+            hipLaunchKernelGGL(batch_c_copy_kernel<double>, dim3(g), dim3(256), 0, st, plan->d_descs,
+                               plan->ndesc, d_offs, (double*)scratch, to_scratch);
+        else
+            hipLaunchKernelGGL(batch_c_copy_kernel<float>, dim3(g), dim3(256), 0, st, plan->d_descs,
+                               plan->ndesc, d_offs, (float*)scratch, to_scratch);
+        HIP_CHECK(hipGetLastError());
+    };
+    copy_c(1);
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    const int keep = plan->mode;
+    int best = keep;
+    float best_ms = 0.f;
+    for (int mode = 0; mode < NMODE; ++mode) {
+        plan->mode = mode;
+        // The general-stride problems run as plain GEMMs whatever the mode;
+        // they are left out of the timing (and so are not re-applied).
+        plan_launch(plan, st, false);             // warm-up (code load)
+        HIP_CHECK(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) plan_launch(plan, st, false);
+        HIP_CHECK(hipEventRecord(e1, st));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (mode == 0 || ms < best_ms) { best_ms = ms; best = mode; }
+    }
+    HIP_CHECK(hipEventDestroy(e0));
+    HIP_CHECK(hipEventDestroy(e1));
+    copy_c(0);
+    HIP_CHECK(hipStreamSynchronize(st));
+    plan->mode = best;
+    if (best != keep && plan->exec) {       // captured graph belongs to the old mode
+        HIP_CHECK(hipGraphExecDestroy(plan->exec));
+        HIP_CHECK(hipGraphDestroy(plan->graph));
+        plan->exec = nullptr;
+        plan->graph = nullptr;
+    }
+    return best;
+}
+
+__attribute__((visibility("default")))
+void bli_amd_gemm_batch_plan_set_mode(void* handle, int mode) {
+    auto* plan = (BatchPlan*)handle;
+    std::lock_guard<std::mutex> lock(api_mutex());
+    if (mode < 0 || mode >= NMODE) abort_with(BLIS_INVALID_UKR_ID, "batch plan mode");
+    if (mode != plan->mode && plan->exec) {
+        HIP_CHECK(hipGraphExecDestroy(plan->exec));
+        HIP_CHECK(hipGraphDestroy(plan->graph));
+        plan->exec = nullptr;
+        plan->graph = nullptr;
+    }
+    plan->mode = mode;
+}
+
+__attribute__((visibility("default")))
+int bli_amd_gemm_batch_plan_get_mode(void* handle) {
+    return ((BatchPlan*)handle)->mode;
 }
 
 __attribute__((visibility("default")))

@@ -30,6 +30,8 @@
 // stores.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
@@ -106,6 +108,102 @@ struct TileProblem {
     int      vec_a, vec_b, vec_c;   // 16-byte vector access allowed
 };
 
+// Addressing of one operand in an edge tile (or a partial last k slab),
+// computed once per tile. The operand is X_eff, rows x K, with
+// X_eff(r, p) = x[p + r*ld] when K_CONTIG, else x[r + p*ld]; the tile covers
+// rows [row0, row0 + ROWS). Each thread owns NV 16-byte vectors of a slab,
+// in the same thread -> element order as the fast path and the LDS stores.
+// Elements outside the matrix are loaded from a clamped address and replaced
+// by zero, so no load sits behind a per-lane branch. A wave uses 16-byte
+// loads whenever none of its lanes straddles the matrix edge (a wave-uniform
+// vote), otherwise element loads.
+template <typename T, class TL, int ROWS, bool K_CONTIG, int NV>
+struct EdgeState {
+    using vec_t = typename Mma<T>::vec_t;
+    static constexpr int V = TL::V;
+    const T* x;
+    int64_t ld, K, klast;          // klast = (K-1)*ld (R_CONTIG)
+    bool vec;                      // 16-byte access allowed (aligned x, ld % V == 0)
+    int64_t rows;
+    int64_t off[NV];               // K_CONTIG: clamped row * ld; R_CONTIG: kk * ld
+    int sub[NV];                   // K_CONTIG: k offset of the vector; R_CONTIG: kk
+    int r0[NV];                    // R_CONTIG: first row of the vector
+    unsigned rmask[NV];            // rows inside the matrix (bit e for element e)
+    bool vecw[NV];                 // R_CONTIG: the whole wave may use vector loads
+
+    __device__ inline void init(const T* x_, int64_t rows_, int64_t K_, int64_t ld_, int vec_,
+                                int64_t row0, int tid) {
+        x = x_; ld = ld_; K = K_; vec = vec_ != 0; rows = rows_;
+        klast = (K - 1) * ld;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const int v = tid + q * TL::NT;
+            if constexpr (K_CONTIG) {
+                const int kc = v % (TL::BK / V), ii = v / (TL::BK / V);
+                const int64_t r = row0 + ii;
+                off[q] = (r < rows ? r : rows - 1) * ld;
+                sub[q] = kc * V;
+                rmask[q] = (r < rows) ? 0xFu : 0u;
+            } else {
+                const int ic = v % (ROWS / V), kk = v / (ROWS / V);
+                const int64_t r = row0 + (int64_t)ic * V;
+                off[q] = (int64_t)kk * ld;
+                sub[q] = kk;
+                unsigned m = 0;
+#pragma unroll
+                for (int e = 0; e < V; ++e) m |= (r + e < rows ? 1u : 0u) << e;
+                rmask[q] = m;
+                r0[q] = (int)r;
+                const bool full = r + V <= rows, none = r >= rows && rows >= V;
+                vecw[q] = __all((vec && (full || none)) ? 1 : 0) != 0;
+            }
+        }
+    }
+
+    template <int NR>
+    __device__ inline void load(vec_t (&r)[NR], int64_t k0) const {
+        static_assert(NR == NV, "register count");
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            if constexpr (K_CONTIG) {
+                const int64_t pk = k0 + sub[q];
+                const T* row = x + off[q];
+                if (vec && k0 + TL::BK <= K) {
+                    const vec_t t = *reinterpret_cast<const vec_t*>(row + pk);
+#pragma unroll
+                    for (int e = 0; e < V; ++e) r[q][e] = rmask[q] ? t[e] : T(0);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e) {
+                        const bool kin = pk + e < K;
+                        const T val = row[kin ? pk + e : K - 1];
+                        r[q][e] = (rmask[q] && kin) ? val : T(0);
+                    }
+                }
+            } else {
+                const bool kin = k0 + sub[q] < K;
+                const T* col = x + (kin ? k0 * ld + off[q] : klast);
+                if (vecw[q]) {
+                    // rows r0 .. r0+V-1 all inside (vector start r0), or all
+                    // outside (any aligned in-bounds vector: start 0)
+                    const int rb = (rmask[q] & 1u) ? r0[q] : 0;
+                    const vec_t t = *reinterpret_cast<const vec_t*>(col + rb);
+#pragma unroll
+                    for (int e = 0; e < V; ++e)
+                        r[q][e] = (kin && ((rmask[q] >> e) & 1u)) ? t[e] : T(0);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e) {
+                        const int64_t ri = r0[q] + e;
+                        const T val = col[ri < rows ? ri : rows - 1];
+                        r[q][e] = (kin && ((rmask[q] >> e) & 1u)) ? val : T(0);
+                    }
+                }
+            }
+        }
+    }
+};
+
 // Registers holding one slab of A between its global load and LDS store.
 template <typename T, class TL, bool A_K>
 struct ALoader {
@@ -113,7 +211,8 @@ struct ALoader {
     static constexpr int V = TL::V;
     vec_t r[TL::NVA];
 
-    __device__ inline void load(const TileProblem<T>& p, int64_t m0, int64_t k0, int tid) {
+    // Edge tile or partial slab, element by element behind bounds checks.
+    __device__ inline void load_guarded(const TileProblem<T>& p, int64_t m0, int64_t k0, int tid) {
 #pragma unroll
         for (int q = 0; q < TL::NVA; ++q) {
             const int v = tid + q * TL::NT;
@@ -187,7 +286,8 @@ struct BLoader {
     static constexpr int V = TL::V;
     vec_t r[TL::NVB];
 
-    __device__ inline void load(const TileProblem<T>& p, int64_t n0, int64_t k0, int tid) {
+    // Edge tile or partial slab, element by element behind bounds checks.
+    __device__ inline void load_guarded(const TileProblem<T>& p, int64_t n0, int64_t k0, int tid) {
 #pragma unroll
         for (int q = 0; q < TL::NVB; ++q) {
             const int v = tid + q * TL::NT;
@@ -362,8 +462,18 @@ __device__ inline void store_tile(const TileProblem<T>& p,
 // must hold 2*TL::STAGE elements. The caller synchronises before reusing smem.
 // FAST: the tile lies inside the matrix and A, B (and C) allow 16-byte access,
 // so every full k slab is loaded without bounds checks.
-template <typename T, class TL, bool A_K, bool B_K, bool FAST>
+// PF: k slabs whose global loads are in flight while a slab is multiplied
+// (1: one register buffer; 2: two, for tiles that stream their operands
+// from HBM, such as the small problems of a batch).
+// EDGE: how edge tiles load. false: element loads behind bounds checks
+// (fewest registers: the single-GEMM kernels, whose occupancy was tuned with
+// it); true: addressing precomputed once per tile (EdgeState) and 16-byte
+// loads wherever a whole wave is inside the matrix (fewer instructions per
+// slab: the batch kernels, where almost every tile is an edge tile).
+// The loaded values, and so the arithmetic, are the same in every variant.
+template <typename T, class TL, bool A_K, bool B_K, bool FAST, int PF, bool EDGE>
 __device__ inline void gemm_tile_impl(const TileProblem<T>& p, int64_t m0, int64_t n0, T* smem) {
+    static_assert(PF == 1 || PF == 2, "prefetch depth");
     using acc_t = typename Mma<T>::acc_t;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -376,36 +486,98 @@ __device__ inline void gemm_tile_impl(const TileProblem<T>& p, int64_t m0, int64
 #pragma unroll
         for (int j = 0; j < TL::TN; ++j) acc[i][j] = acc_t{};
 
-    ALoader<T, TL, A_K> al;
-    BLoader<T, TL, B_K> bl;
     const int64_t nk = (p.k + TL::BK - 1) / TL::BK;
-
-    auto load = [&](int64_t k0) {
+    using EA = EdgeState<T, TL, TL::BM, A_K, TL::NVA>;
+    using EB = EdgeState<T, TL, TL::BN, B_K, TL::NVB>;
+    // With EDGE, edge tiles keep their addressing for every slab; interior
+    // (FAST) tiles build it only for a partial last slab, so it holds no
+    // registers during their main loop.
+    EA ea;
+    EB eb;
+    if constexpr (EDGE && !FAST) {
+        if (nk > 0) {
+            ea.init(p.a, p.m, p.k, p.lda, p.vec_a, m0, tid);
+            eb.init(p.b, p.n, p.k, p.ldb, p.vec_b, n0, tid);
+        }
+    }
+    auto load = [&](ALoader<T, TL, A_K>& al, BLoader<T, TL, B_K>& bl, int64_t k0) {
         if (FAST && k0 + TL::BK <= p.k) {
             al.load_fast(p, m0, k0, tid);
             bl.load_fast(p, n0, k0, tid);
+        } else if constexpr (!EDGE) {
+            al.load_guarded(p, m0, k0, tid);
+            bl.load_guarded(p, n0, k0, tid);
+        } else if constexpr (FAST) {
+            EA ta;
+            EB tb;
+            ta.init(p.a, p.m, p.k, p.lda, p.vec_a, m0, tid);
+            tb.init(p.b, p.n, p.k, p.ldb, p.vec_b, n0, tid);
+            ta.load(al.r, k0);
+            tb.load(bl.r, k0);
         } else {
-            al.load(p, m0, k0, tid);
-            bl.load(p, n0, k0, tid);
+            ea.load(al.r, k0);
+            eb.load(bl.r, k0);
         }
     };
 
-    load(0);
-    al.store(smem, tid);
-    bl.store(smem + TL::A_ELEMS, tid);
-    __syncthreads();
-
-    for (int64_t s = 0; s < nk; ++s) {
-        T* cur = smem + (s & 1) * TL::STAGE;
-        T* nxt = smem + ((s + 1) & 1) * TL::STAGE;
-        const bool more = (s + 1) < nk;
-        if (more) load((s + 1) * TL::BK);
-        mma_slab<T, TL>(cur, cur + TL::A_ELEMS, acc, wm0, wn0, lane);
-        if (more) {
-            al.store(nxt, tid);
-            bl.store(nxt + TL::A_ELEMS, tid);
+    if constexpr (PF == 1) {
+        ALoader<T, TL, A_K> al;
+        BLoader<T, TL, B_K> bl;
+        // k == 0 (a batch problem with k == 0 or alpha == 0) touches neither
+        // A nor B: the loads clamp addresses to the last row/column, which
+        // exists only when k >= 1.
+        if (nk > 0) {
+            load(al, bl, 0);
+            al.store(smem, tid);
+            bl.store(smem + TL::A_ELEMS, tid);
         }
         __syncthreads();
+        for (int64_t s = 0; s < nk; ++s) {
+            T* cur = smem + (s & 1) * TL::STAGE;
+            T* nxt = smem + ((s + 1) & 1) * TL::STAGE;
+            const bool more = (s + 1) < nk;
+            if (more) load(al, bl, (s + 1) * TL::BK);
+            mma_slab<T, TL>(cur, cur + TL::A_ELEMS, acc, wm0, wn0, lane);
+            if (more) {
+                al.store(nxt, tid);
+                bl.store(nxt + TL::A_ELEMS, tid);
+            }
+            __syncthreads();
+        }
+    } else {
+        // Two register buffers. At step s, buffer (s & 1) has already been
+        // written to LDS and now receives slab s + 2; buffer ((s + 1) & 1)
+        // holds slab s + 1 and is written to the other LDS stage after the
+        // multiply. The loop is unrolled by two so that each buffer is a
+        // fixed set of registers.
+        ALoader<T, TL, A_K> a0, a1;
+        BLoader<T, TL, B_K> b0, b1;
+        if (nk > 0) {
+            load(a0, b0, 0);
+            if (nk > 1) load(a1, b1, TL::BK);
+            a0.store(smem, tid);
+            b0.store(smem + TL::A_ELEMS, tid);
+        }
+        __syncthreads();
+        auto step = [&](auto buf, int64_t s) {
+            constexpr int B = decltype(buf)::value;   // == s & 1
+            T* cur = smem + B * TL::STAGE;
+            T* nxt = smem + (1 - B) * TL::STAGE;
+            if (s + 2 < nk) {
+                if constexpr (B == 0) load(a0, b0, (s + 2) * TL::BK);
+                else load(a1, b1, (s + 2) * TL::BK);
+            }
+            mma_slab<T, TL>(cur, cur + TL::A_ELEMS, acc, wm0, wn0, lane);
+            if (s + 1 < nk) {
+                if constexpr (B == 0) { a1.store(nxt, tid); b1.store(nxt + TL::A_ELEMS, tid); }
+                else { a0.store(nxt, tid); b0.store(nxt + TL::A_ELEMS, tid); }
+            }
+            __syncthreads();
+        };
+        for (int64_t s = 0; s < nk; s += 2) {
+            step(std::integral_constant<int, 0>{}, s);
+            if (s + 1 < nk) step(std::integral_constant<int, 1>{}, s + 1);
+        }
     }
 
     if (FAST && p.vec_c)
@@ -414,12 +586,12 @@ __device__ inline void gemm_tile_impl(const TileProblem<T>& p, int64_t m0, int64
         store_tile<T, TL, false>(p, acc, m0 + wm0, n0 + wn0, lane);
 }
 
-template <typename T, class TL, bool A_K, bool B_K>
+template <typename T, class TL, bool A_K, bool B_K, int PF = 1, bool EDGE = false>
 __device__ inline void gemm_tile(const TileProblem<T>& p, int64_t tm, int64_t tn, T* smem) {
     const int64_t m0 = tm * TL::BM, n0 = tn * TL::BN;
     const bool fast = p.vec_a && p.vec_b && m0 + TL::BM <= p.m && n0 + TL::BN <= p.n;
-    if (fast) gemm_tile_impl<T, TL, A_K, B_K, true>(p, m0, n0, smem);
-    else gemm_tile_impl<T, TL, A_K, B_K, false>(p, m0, n0, smem);
+    if (fast) gemm_tile_impl<T, TL, A_K, B_K, true, PF, EDGE>(p, m0, n0, smem);
+    else gemm_tile_impl<T, TL, A_K, B_K, false, PF, EDGE>(p, m0, n0, smem);
 }
 
 } // namespace dev

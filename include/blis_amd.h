@@ -59,6 +59,13 @@ void* bli_amd_sgemm_batch_plan_create( const trans_t* transa, const trans_t* tra
 void  bli_amd_gemm_batch_plan_execute( void* plan, int use_graph );
 int   bli_amd_gemm_batch_plan_work_items( void* plan );
 void  bli_amd_gemm_batch_plan_destroy( void* plan );
+/* Batch schedule autotuner: times every kernel mode of the plan (k slabs in
+   flight, edge-tile loading) on the plan's own problems, keeps the fastest
+   and returns it. C is saved and restored, so the caller's data is left as
+   it was. All modes compute bit-identical results. */
+int   bli_amd_gemm_batch_plan_autotune( void* plan, int reps );
+void  bli_amd_gemm_batch_plan_set_mode( void* plan, int mode );
+int   bli_amd_gemm_batch_plan_get_mode( void* plan );
 
 /* ---- split-precision GEMM ----------------------------------------------
    Same arguments and semantics as bli_?gemm (C := beta*C + alpha*op(A)*op(B);

@@ -168,3 +168,86 @@ def test_dgemm_batch_f77_groups(blis, oracle):
                     b.view(), g["beta"], ref)
         err = normwise_gemm_err(c.device_view(), ref, a.view(), b.view(), c0, g["alpha"], g["beta"])
         assert err <= 4 * g["k"] * unit_roundoff(np.float64)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batch_trivial_problems(blis, dtype):
+    """k == 0 and alpha == 0 problems inside a batch: C := beta*C, A and B
+    are not read (their pointers are null here); m == 1 / n == 1 problems."""
+    ch = "d" if np.dtype(dtype) == np.float64 else "s"
+    say(f"{ch}gemm_batch trivial problems")
+    probs = make_batch(dtype, 12, seed=21, lo=1, hi=70)
+    probs[0].update(k=0)
+    probs[3].update(alpha=0.0)
+    probs[5].update(m=1)
+    probs[7].update(n=1)
+    for p in probs:
+        if p is probs[5] or p is probs[7]:
+            p["c"] = Mat(p["m"], p["n"], rs=1, cs=max(p["m"], 1), dtype=dtype, seed=77)
+            p["c0"] = p["c"].view().copy()
+            am, an = (p["k"], p["m"]) if p["ta"] == "t" else (p["m"], p["k"])
+            bm, bn = (p["n"], p["k"]) if p["tb"] == "t" else (p["k"], p["n"])
+            p["a"] = Mat(am, an, dtype=dtype, seed=78)
+            p["b"] = Mat(bm, bn, dtype=dtype, seed=79)
+        for x in "abc":
+            p[x].to_device()
+    args, keep = typed_batch_args(probs, dtype)
+    # null A and B for the k == 0 and alpha == 0 problems
+    aptrs = C.cast(args[6], C.POINTER(C.c_void_p))
+    bptrs = C.cast(args[9], C.POINTER(C.c_void_p))
+    for i in (0, 3):
+        aptrs[i] = None
+        bptrs[i] = None
+    getattr(blis, f"bli_{ch}gemm_batch")(*args)
+    u = unit_roundoff(dtype)
+    for i, p in enumerate(probs):
+        got = p["c"].device_view()
+        if i in (0, 3):
+            np.testing.assert_array_equal(got, (p["beta"] * p["c0"]).astype(dtype))
+        else:
+            a = p["a"].view().astype(np.float64)
+            b = p["b"].view().astype(np.float64)
+            ref = p["alpha"] * ((a.T if p["ta"] == "t" else a) @ (b.T if p["tb"] == "t" else b)) \
+                + p["beta"] * p["c0"].astype(np.float64)
+            err = normwise_gemm_err(got, ref, a, b, p["c0"], p["alpha"], p["beta"])
+            assert err <= 4 * max(p["k"], 1) * u, (i, err)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batch_modes_and_autotune(blis, dtype):
+    """Every kernel mode gives bit-identical results; the autotuner leaves C
+    as it found it and picks a valid mode."""
+    ch = "d" if np.dtype(dtype) == np.float64 else "s"
+    say(f"{ch}gemm_batch modes + autotune")
+    probs = make_batch(dtype, 80, seed=31, lo=5, hi=300)
+    for p in probs:
+        p["beta"] = 0.0            # each execute is then idempotent
+        for x in "abc":
+            p[x].to_device()
+    args, keep = typed_batch_args(probs, dtype)
+    plan = getattr(blis, f"bli_amd_{ch}gemm_batch_plan_create")(*args)
+    try:
+        results = []
+        for mode in range(5):
+            blis.bli_amd_gemm_batch_plan_set_mode(plan, mode)
+            assert blis.bli_amd_gemm_batch_plan_get_mode(plan) == mode
+            for p in probs:
+                p["c"].dbuf.fill_(7.0)
+            blis.bli_amd_gemm_batch_plan_execute(plan, mode % 2)
+            results.append([p["c"].device_view().copy() for p in probs])
+        for mode in range(1, 5):
+            for r0, r in zip(results[0], results[mode]):
+                np.testing.assert_array_equal(r, r0)
+        # the autotuner restores C
+        for p in probs:
+            p["c"].dbuf.fill_(3.0)
+        before = [p["c"].device_view().copy() for p in probs]
+        tuned = blis.bli_amd_gemm_batch_plan_autotune(plan, 2)
+        assert 0 <= tuned < 5 and blis.bli_amd_gemm_batch_plan_get_mode(plan) == tuned
+        for p, b in zip(probs, before):
+            np.testing.assert_array_equal(p["c"].device_view(), b)
+        blis.bli_amd_gemm_batch_plan_execute(plan, 1)
+        for p, r0 in zip(probs, results[0]):
+            np.testing.assert_array_equal(p["c"].device_view(), r0)
+    finally:
+        blis.bli_amd_gemm_batch_plan_destroy(plan)
