@@ -77,8 +77,13 @@ struct TileCfg {
     static constexpr int NT = 64 * NW;            // threads per workgroup
     static constexpr int TM = WM / 16, TN = WN / 16;
     static constexpr int V = M::V;
-    static constexpr int LDA = BM + V;            // LDS row pitch (16 B pad)
-    static constexpr int LDB = BN + V;
+    // LDS row pitch: rows are padded by one 16-byte vector (a one-element
+    // pad for FP64 was measured to raise bank conflicts: it costs the 16-byte
+    // alignment of the fragment reads).
+    static constexpr int PAD = V;
+    static constexpr bool LDS_VEC = (PAD % V) == 0;   // rows start 16-byte aligned
+    static constexpr int LDA = BM + PAD;
+    static constexpr int LDB = BN + PAD;
     static constexpr int A_ELEMS = BK * LDA;
     static constexpr int B_ELEMS = BK * LDB;
     static constexpr int STAGE = A_ELEMS + B_ELEMS;
@@ -270,7 +275,12 @@ struct ALoader {
             const int v = tid + q * TL::NT;
             if constexpr (!A_K) {
                 const int ic = v % (TL::BM / V), kk = v / (TL::BM / V);
-                *reinterpret_cast<vec_t*>(As + kk * TL::LDA + ic * V) = r[q];
+                if constexpr (TL::LDS_VEC) {
+                    *reinterpret_cast<vec_t*>(As + kk * TL::LDA + ic * V) = r[q];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e) As[kk * TL::LDA + ic * V + e] = r[q][e];
+                }
             } else {
                 const int kc = v % (TL::BK / V), ii = v / (TL::BK / V);
 #pragma unroll
@@ -347,7 +357,12 @@ struct BLoader {
                 for (int e = 0; e < V; ++e) Bs[(kc * V + e) * TL::LDB + jj] = r[q][e];
             } else {
                 const int jc = v % (TL::BN / V), kk = v / (TL::BN / V);
-                *reinterpret_cast<vec_t*>(Bs + kk * TL::LDB + jc * V) = r[q];
+                if constexpr (TL::LDS_VEC) {
+                    *reinterpret_cast<vec_t*>(Bs + kk * TL::LDB + jc * V) = r[q];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e) Bs[kk * TL::LDB + jc * V + e] = r[q][e];
+                }
             }
         }
     }
@@ -366,7 +381,7 @@ __device__ inline void mma_slab(const T* As, const T* Bs,
         const T* ap = As + (ks + lk) * TL::LDA + wm0 + TL::TM * lr;
         const T* bp = Bs + (ks + lk) * TL::LDB + wn0 + TL::TN * lr;
         T af[TL::TM], bf[TL::TN];
-        if constexpr (TL::TM % V == 0) {
+        if constexpr (TL::TM % V == 0 && TL::LDS_VEC) {
 #pragma unroll
             for (int q = 0; q < TL::TM / V; ++q) {
                 vec_t t = *reinterpret_cast<const vec_t*>(ap + q * V);
@@ -377,7 +392,7 @@ __device__ inline void mma_slab(const T* As, const T* Bs,
 #pragma unroll
             for (int i = 0; i < TL::TM; ++i) af[i] = ap[i];
         }
-        if constexpr (TL::TN % V == 0) {
+        if constexpr (TL::TN % V == 0 && TL::LDS_VEC) {
 #pragma unroll
             for (int q = 0; q < TL::TN / V; ++q) {
                 vec_t t = *reinterpret_cast<const vec_t*>(bp + q * V);

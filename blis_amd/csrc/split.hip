@@ -273,6 +273,13 @@ struct SliceTile {
     static constexpr int LDS = 2 * OPB;
     static constexpr int ROUNDS = 64 * CPR / NT;        // load instructions per wave per operand
     static_assert(ROUNDS == S, "each wave issues S loads per operand");
+    // Two workgroups per CU overlap each other's loads and multiplies. Each
+    // also double-buffers its LDS tile when two stages of both workgroups
+    // fit (S <= 5); above that a single stage is used (measured at S = 8:
+    // one double-buffered workgroup per CU was 16% slower than two single-
+    // buffered ones).
+    static constexpr int NBUF = (4 * LDS <= 160 * 1024) ? 2 : 1;
+    static constexpr int WAVES_PER_EU = S <= 8 ? 2 : 1;
 };
 
 // Position of logical chunk c of row r inside the LDS row (an involution).
@@ -300,10 +307,11 @@ struct SliceGemmArgs {
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
 template <typename T, int S>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 8 ? 2 : 1, S <= 8 ? 2 : 1)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SliceTile<S>::WAVES_PER_EU,
+                                                                     SliceTile<S>::WAVES_PER_EU)))
 void slice_gemm_kernel(SliceGemmArgs g) {
     using TL = SliceTile<S>;
-    __shared__ __attribute__((aligned(16))) signed char smem[TL::LDS];
+    __shared__ __attribute__((aligned(16))) signed char smem[TL::NBUF * TL::LDS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wx = (wave & 1) * 32, wy = (wave >> 1) * 32;
     const int64_t x0 = (int64_t)blockIdx.x * 64, y0 = (int64_t)blockIdx.y * 64;
@@ -325,12 +333,12 @@ void slice_gemm_kernel(SliceGemmArgs g) {
         const int r = P / TL::CPR, cp = P % TL::CPR;
         src[q] = (r * TL::CPR + swz<S>(r, cp)) * 16;
     }
-    auto stage = [&](int64_t kc) {
+    auto stage = [&](int64_t kc, int buf) {
         const signed char* gx = g.xs + ((kc * g.x_pad + x0) * S) * KC;
         const signed char* gy = g.ys + ((kc * g.y_pad + y0) * S) * KC;
 #pragma unroll
         for (int q = 0; q < S; ++q) {
-            signed char* dx = smem + (q * 4 + wave) * 64 * 16;
+            signed char* dx = smem + buf * TL::LDS + (q * 4 + wave) * 64 * 16;
             __builtin_amdgcn_global_load_lds((const void*)(gx + src[q]), (lds_ptr_t)dx, 16, 0, 0);
             __builtin_amdgcn_global_load_lds((const void*)(gy + src[q]), (lds_ptr_t)(dx + TL::OPB),
                                              16, 0, 0);
@@ -338,12 +346,14 @@ void slice_gemm_kernel(SliceGemmArgs g) {
     };
 
     const int lr = lane & 15, kk = lane >> 4;
-    stage(g.kc0);
+    stage(g.kc0, 0);
     for (int64_t kc = g.kc0; kc < g.kc1; ++kc) {
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        const signed char* sx = smem;
-        const signed char* sy = smem + TL::OPB;
+        const int buf = (TL::NBUF == 2) ? (int)((kc - g.kc0) & 1) : 0;
+        __builtin_amdgcn_s_waitcnt(0);   // this wave's loads of chunk kc
+        __syncthreads();                 // everyone's loads; the other stage is free
+        if (TL::NBUF == 2 && kc + 1 < g.kc1) stage(kc + 1, buf ^ 1);
+        const signed char* sx = smem + buf * TL::LDS;
+        const signed char* sy = sx + TL::OPB;
         i4_t yf[S][2];
 #pragma unroll
         for (int u = 0; u < S; ++u)
@@ -369,8 +379,10 @@ void slice_gemm_kernel(SliceGemmArgs g) {
                         acc[t + u][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
                             xf[i], yf[u][j], acc[t + u][i][j], 0, 0, 0);
         }
-        __syncthreads();                 // every wave is done with this chunk
-        if (kc + 1 < g.kc1) stage(kc + 1);
+        if (TL::NBUF == 1) {
+            __syncthreads();             // every wave is done with this chunk
+            if (kc + 1 < g.kc1) stage(kc + 1, 0);
+        }
     }
 
     // Epilogue: class c (weight d = c + 2) carries 2^-7(c+2).
