@@ -383,6 +383,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-split", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--only-batch", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one 1 GiB device copy, then two headline DGEMM calls (for "
                          "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE calibration)")
@@ -396,6 +397,12 @@ def main():
     bench = Bench()
     if args.only_split:
         print(json.dumps(split_bench(bench, log)), flush=True)
+        return
+    if args.only_batch:
+        res = {}
+        for ch, key in (("d", "f64"), ("s", "f32")):
+            res.update(batch_bench(bench, ch, key, log))
+        print(json.dumps(res), flush=True)
         return
     if args.traffic_probe:
         # Calibration: a plain device copy of exactly 1 GiB read + 1 GiB written.

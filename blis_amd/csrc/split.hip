@@ -306,6 +306,8 @@ struct SliceGemmArgs {
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
+constexpr int SPLIT_GROUP = 8;   // raster group height in tiles
+
 template <typename T, int S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SliceTile<S>::WAVES_PER_EU,
                                                                      SliceTile<S>::WAVES_PER_EU)))
@@ -314,7 +316,24 @@ void slice_gemm_kernel(SliceGemmArgs g) {
     __shared__ __attribute__((aligned(16))) signed char smem[TL::NBUF * TL::LDS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wx = (wave & 1) * 32, wy = (wave >> 1) * 32;
-    const int64_t x0 = (int64_t)blockIdx.x * 64, y0 = (int64_t)blockIdx.y * 64;
+    // 1-D grid over the (x_pad/64) x (y_pad/64) tiles. Workgroups are
+    // dispatched round-robin over the 8 XCDs; each XCD gets a contiguous
+    // range of logical tiles, rasterised in groups of SPLIT_GROUP tile rows,
+    // so the slice tiles a workgroup streams are shared in its XCD's L2.
+    int64_t x0, y0;
+    {
+        const int nb = gridDim.x, gx = (int)(g.x_pad / 64), gy = (int)(g.y_pad / 64);
+        int bid = blockIdx.x;
+        const int q = nb / 8, rm = nb % 8, xcd = bid % 8, l = bid / 8;
+        bid = xcd * q + (xcd < rm ? xcd : rm) + l;
+        const int per_group = SPLIT_GROUP * gy;
+        const int group = bid / per_group;
+        const int first = group * SPLIT_GROUP;
+        const int gsz = (gx - first < SPLIT_GROUP) ? gx - first : SPLIT_GROUP;
+        const int in = bid - group * per_group;
+        x0 = (int64_t)(first + in % gsz) * 64;
+        y0 = (int64_t)(in / gsz) * 64;
+    }
 
     i4_t acc[S][2][2];
 #pragma unroll
@@ -479,7 +498,7 @@ static void launch_slice(Dt dt, View v, int64_t rows, int64_t rows_pad, int64_t 
 
 template <typename T>
 static void launch_slice_gemm(int S, const SliceGemmArgs& a, hipStream_t st) {
-    const dim3 grid((unsigned)(a.x_pad / 64), (unsigned)(a.y_pad / 64));
+    const dim3 grid((unsigned)((a.x_pad / 64) * (a.y_pad / 64)));
     switch (S) {
 #define BLIS_AMD_SLICE_CASE(n) \
     case n: hipLaunchKernelGGL((slice_gemm_kernel<T, n>), grid, dim3(256), 0, st, a); break;
