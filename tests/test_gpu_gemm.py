@@ -216,3 +216,61 @@ def test_gemm_large_probe_residual(blis, dtype, shape):
         assert r <= 4 * k * unit_roundoff(dtype), (ta, tb, r)
         del a, b, c
         torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_gemm_k_contiguous_operands_match_rows_contiguous(blis, dtype):
+    """Large GEMMs move a k-contiguous operand (op(A) = A^T, op(B) = B) to the
+    rows-contiguous layout before the tile kernel runs. The result must be
+    bit-identical to the same product given with the other layout, at ragged
+    sizes and padded leading dimensions (the 64 x 64 transpose tiles have
+    partial edges), and close to an FP64 product on sampled rows."""
+    import torch
+    m, n, k = 6000, 4100, 3001
+    dev = torch.device("cuda")
+    g = torch.Generator(device="cpu").manual_seed(77)
+    A = (torch.rand(m, k, generator=g, dtype=torch.float64) * 2 - 1).to(dtype=_torch_dt(dtype))
+    B = (torch.rand(k, n, generator=g, dtype=torch.float64) * 2 - 1).to(dtype=_torch_dt(dtype))
+    C0 = (torch.rand(m, n, generator=g, dtype=torch.float64) * 2 - 1).to(dtype=_torch_dt(dtype))
+
+    def colmajor(x, pad):
+        # column-major storage with leading dimension rows + pad
+        buf = torch.zeros(x.shape[1], x.shape[0] + pad, dtype=x.dtype)
+        buf[:, :x.shape[0]] = x.t()
+        return buf.to(dev), x.shape[0] + pad
+
+    T = _ct(dtype)
+    ch = "d" if np.dtype(dtype) == np.float64 else "s"
+    al, be = T(0.75), T(-0.5)
+    outs = {}
+    for case in ("nn", "tn", "nt"):
+        say(f"k-contiguous layouts {ch} {case} {m}x{n}x{k}")
+        if case[0] == "n":
+            a, lda = colmajor(A, 5)
+        else:
+            a, lda = colmajor(A.t().contiguous(), 5)
+        if case[1] == "n":
+            b, ldb = colmajor(B, 3)
+        else:
+            b, ldb = colmajor(B.t().contiguous(), 3)
+        c, ldc = colmajor(C0, 0)
+        getattr(blis, f"bli_{ch}gemm")(TR[case[0]], TR[case[1]], m, n, k, C.byref(al),
+                                       a.data_ptr(), 1, lda, b.data_ptr(), 1, ldb, C.byref(be),
+                                       c.data_ptr(), 1, ldc)
+        torch.cuda.synchronize()
+        outs[case] = c[:, :m].cpu().t().numpy().copy()
+        del a, b, c
+    assert np.array_equal(outs["nn"], outs["nt"])
+    assert np.array_equal(outs["tn"], outs["nt"])
+    rows = np.arange(0, m, 97)
+    An, Bn, Cn = (x.double().numpy() for x in (A, B, C0))
+    ref = 0.75 * (An[rows] @ Bn) - 0.5 * Cn[rows]
+    err = np.linalg.norm(outs["nn"][rows] - ref) / (
+        0.75 * np.linalg.norm(An[rows]) * np.linalg.norm(Bn) + 0.5 * np.linalg.norm(Cn[rows]))
+    assert err <= 4 * k * unit_roundoff(dtype), err
+    torch.cuda.empty_cache()
+
+
+def _torch_dt(dtype):
+    import torch
+    return torch.float64 if np.dtype(dtype) == np.float64 else torch.float32
