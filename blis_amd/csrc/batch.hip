@@ -77,23 +77,46 @@ struct XcdSegs {
 //   1: one slab in flight, precomputed edge addressing (EdgeState);
 //   2: two slabs in flight, precomputed edge addressing;
 //   3: two slabs in flight, guarded element loads;
-//   4: wave tiles (gemm_batch_wave_kernel).
-constexpr int NMODE = 5;
+//   4: wave tiles (gemm_batch_wave_kernel);
+//   5: as 0, with registers capped for four workgroups per CU and C written
+//      through LDS in whole cache lines;
+//   6: as 0, with registers capped for four workgroups per CU.
+// Modes other than 4 can differ between the variants of one plan.
+constexpr int NMODE = 7;
 constexpr int WAVE_MODE = 4;
-constexpr int BATCH_DEFAULT_MODE = 0;
-constexpr int mode_pf(int mode) { return mode >= 2 ? 2 : 1; }
+// Mode used until the autotuner runs: the one measured fastest on uneven
+// batches of 10,000 problems (dims log-uniform in [8, 512]) for each type.
+constexpr int default_mode(Dt dt) { return dt == Dt::F64 ? 6 : 5; }
+constexpr int mode_pf(int mode) { return (mode == 2 || mode == 3) ? 2 : 1; }
 constexpr bool mode_edge(int mode) { return mode == 1 || mode == 2; }
+constexpr int mode_waves(int mode) { return (mode == 5 || mode == 6) ? 4 : 1; }
+constexpr bool mode_lds_epi(int mode) { return mode == 5; }
 
 template <typename T, class TL, bool A_K, bool B_K, int MODE>
-__global__ __launch_bounds__(TL::NT) void gemm_batch_kernel(const BatchDesc* __restrict__ descs,
+__global__ __launch_bounds__(TL::NT)
+__attribute__((amdgpu_waves_per_eu(mode_waves(MODE)))) void gemm_batch_kernel(const BatchDesc* __restrict__ descs,
                                                             const WorkItem* __restrict__ items,
                                                             XcdSegs seg) {
     __shared__ T smem[2 * TL::STAGE];
     const int x = blockIdx.x % NXCD;
     const int slot = blockIdx.x / NXCD, nslot = gridDim.x / NXCD;   // gridDim.x % NXCD == 0
-    for (int it = seg.off[x] + slot; it < seg.off[x + 1]; it += nslot) {
-        const WorkItem w = items[it];
-        const BatchDesc& d = descs[w.prob];
+    const int end = seg.off[x + 1];
+    int it = seg.off[x] + slot;
+    if (it >= end) return;
+    // The next work item and its descriptor are fetched while the current
+    // tile runs, so a tile's operand loads do not wait behind two dependent
+    // descriptor reads.
+    WorkItem w = items[it];
+    BatchDesc d = descs[w.prob];
+    for (;;) {
+        const int nit = it + nslot;
+        const bool has_next = nit < end;
+        WorkItem wn{};
+        BatchDesc dn{};
+        if (has_next) {
+            wn = items[nit];
+            dn = descs[wn.prob];
+        }
         TileProblem<T> p;
         p.a = (const T*)d.a; p.b = (const T*)d.b; p.c = (T*)d.c;
         p.m = d.m; p.n = d.n; p.k = d.k;
@@ -103,8 +126,13 @@ __global__ __launch_bounds__(TL::NT) void gemm_batch_kernel(const BatchDesc* __r
         p.vec_b = (d.flags & F_VEC_B) != 0;
         p.vec_c = (d.flags & F_VEC_C) != 0;
         const int64_t tm = w.tile % d.tiles_m, tn = w.tile / d.tiles_m;
-        gemm_tile<T, TL, A_K, B_K, mode_pf(MODE), mode_edge(MODE)>(p, tm, tn, smem);
+        gemm_tile<T, TL, A_K, B_K, mode_pf(MODE), mode_edge(MODE), mode_lds_epi(MODE)>(p, tm, tn,
+                                                                                  smem);
         __syncthreads();
+        if (!has_next) break;
+        it = nit;
+        w = wn;
+        d = dn;
     }
 }
 
@@ -244,6 +272,8 @@ inline void variant_info(int mode, int v, const void*& fn, int& threads) {
     case 1: fn = variant_fn<T, 1>(v); return;
     case 2: fn = variant_fn<T, 2>(v); return;
     case 3: fn = variant_fn<T, 3>(v); return;
+    case 5: fn = variant_fn<T, 5>(v); return;
+    case 6: fn = variant_fn<T, 6>(v); return;
     }
     abort_with(BLIS_INVALID_UKR_ID, "batch: mode");
 }
@@ -294,6 +324,7 @@ struct BatchPlan {
     WorkItem* d_witems = nullptr;     // wave-tile mode: 32 x 32 tiles of every problem
     XcdSegs wseg[4] = {};             // per layout variant (2*a_k + b_k)
     int mode = 0;                     // kernel mode used by execute
+    int vmode[NVARIANT] = {};         // per-variant kernel mode (mode != WAVE_MODE)
     int ndesc = 0;
     int64_t c_elems = 0;              // sum of m*n (autotuner scratch)
     std::vector<int64_t> c_offs;
@@ -489,7 +520,8 @@ static BatchPlan* plan_build(Dt dt, const std::vector<BatchProblem>& probs, hipS
             const int slots = std::max(1, std::min(longest, cu_count() * per_cu / NXCD));
             plan->grid[mode][v] = slots * NXCD;
         }
-    plan->mode = BATCH_DEFAULT_MODE;
+    plan->mode = default_mode(dt);
+    for (int v = 0; v < NVARIANT; ++v) plan->vmode[v] = plan->mode;
     plan->ndesc = (int)descs.size();
     plan->c_offs.resize(descs.size());
     for (size_t i = 0; i < descs.size(); ++i) {
@@ -499,20 +531,24 @@ static BatchPlan* plan_build(Dt dt, const std::vector<BatchProblem>& probs, hipS
     return plan;
 }
 
+// Launch variant v of the plan in kernel mode `mode` (nothing if the variant
+// has no work in that mode).
+static void launch_variant(const BatchPlan* plan, int mode, int v, hipStream_t st) {
+    if (plan->grid[mode][v] == 0) return;
+    const void* fn;
+    int threads;
+    if (plan->dt == Dt::F64) variant_info<double>(mode, v, fn, threads);
+    else variant_info<float>(mode, v, fn, threads);
+    const BatchDesc* descs = plan->d_descs;
+    const WorkItem* items = (mode == WAVE_MODE) ? plan->d_witems : plan->d_items;
+    XcdSegs seg = (mode == WAVE_MODE) ? plan->wseg[v] : plan->seg[v];
+    void* args[] = {(void*)&descs, (void*)&items, (void*)&seg};
+    HIP_CHECK(hipLaunchKernel(fn, dim3(plan->grid[mode][v]), dim3(threads), args, 0, st));
+}
+
 static void plan_launch(const BatchPlan* plan, hipStream_t st, bool with_fallback = true) {
-    const int mode = plan->mode;
-    for (int v = 0; v < NVARIANT; ++v) {
-        if (plan->grid[mode][v] == 0) continue;
-        const void* fn;
-        int threads;
-        if (plan->dt == Dt::F64) variant_info<double>(mode, v, fn, threads);
-        else variant_info<float>(mode, v, fn, threads);
-        const BatchDesc* descs = plan->d_descs;
-        const WorkItem* items = (mode == WAVE_MODE) ? plan->d_witems : plan->d_items;
-        XcdSegs seg = (mode == WAVE_MODE) ? plan->wseg[v] : plan->seg[v];
-        void* args[] = {(void*)&descs, (void*)&items, (void*)&seg};
-        HIP_CHECK(hipLaunchKernel(fn, dim3(plan->grid[mode][v]), dim3(threads), args, 0, st));
-    }
+    for (int v = 0; v < NVARIANT; ++v)
+        launch_variant(plan, plan->mode == WAVE_MODE ? WAVE_MODE : plan->vmode[v], v, st);
     if (!with_fallback) return;
     for (const BatchProblem& q : plan->fallback) {
         GemmDesc d;
@@ -643,34 +679,65 @@ int bli_amd_gemm_batch_plan_autotune(void* handle, int reps) {
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
-    const int keep = plan->mode;
-    int best = keep;
-    float best_ms = 0.f;
-    for (int mode = 0; mode < NMODE; ++mode) {
-        plan->mode = mode;
-        // The general-stride problems run as plain GEMMs whatever the mode;
-        // they are left out of the timing (and so are not re-applied).
-        plan_launch(plan, st, false);             // warm-up (code load)
+    // The general-stride problems run as plain GEMMs whatever the mode; they
+    // are left out of the timing (and so are not re-applied).
+    auto time_ms = [&](auto&& launch) {
+        launch();                                   // warm-up (code load)
         HIP_CHECK(hipEventRecord(e0, st));
-        for (int r = 0; r < reps; ++r) plan_launch(plan, st, false);
+        for (int r = 0; r < reps; ++r) launch();
         HIP_CHECK(hipEventRecord(e1, st));
         HIP_CHECK(hipEventSynchronize(e1));
         float ms = 0.f;
         HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        if (mode == 0 || ms < best_ms) { best_ms = ms; best = mode; }
+        return ms / reps;
+    };
+    // Every variant is its own kernel: time each one in each LDS-tile mode
+    // and keep its fastest; the wave-tile mode covers all variants at once
+    // and is timed as a whole.
+    int vbest[NVARIANT];
+    float vbest_ms[NVARIANT];
+    float sum_ms = 0.f;
+    for (int v = 0; v < NVARIANT; ++v) {
+        vbest[v] = default_mode(plan->dt);   // never the wave mode (it spans variants)
+        vbest_ms[v] = 0.f;
+        for (int mode = 0; mode < NMODE; ++mode) {
+            if (mode == WAVE_MODE || plan->grid[mode][v] == 0) continue;
+            const float ms = time_ms([&] { launch_variant(plan, mode, v, st); });
+            if (mode == 0 || ms < vbest_ms[v]) { vbest_ms[v] = ms; vbest[v] = mode; }
+        }
+        sum_ms += vbest_ms[v];
     }
+    const float wave_ms = time_ms([&] {
+        for (int v = 0; v < NVARIANT; ++v) launch_variant(plan, WAVE_MODE, v, st);
+    });
     HIP_CHECK(hipEventDestroy(e0));
     HIP_CHECK(hipEventDestroy(e1));
     copy_c(0);
     HIP_CHECK(hipStreamSynchronize(st));
-    plan->mode = best;
-    if (best != keep && plan->exec) {       // captured graph belongs to the old mode
+    bool changed = false;
+    if (wave_ms < sum_ms) {
+        changed = plan->mode != WAVE_MODE;
+        plan->mode = WAVE_MODE;
+        for (int v = 0; v < NVARIANT; ++v) plan->vmode[v] = WAVE_MODE;
+    } else {
+        // Reported mode: the one of the variant that takes longest.
+        int top = 0;
+        for (int v = 1; v < NVARIANT; ++v)
+            if (vbest_ms[v] > vbest_ms[top]) top = v;
+        changed = plan->mode != vbest[top];
+        plan->mode = vbest[top];
+        for (int v = 0; v < NVARIANT; ++v) {
+            changed = changed || plan->vmode[v] != vbest[v];
+            plan->vmode[v] = vbest[v];
+        }
+    }
+    if (changed && plan->exec) {            // captured graph belongs to the old modes
         HIP_CHECK(hipGraphExecDestroy(plan->exec));
         HIP_CHECK(hipGraphDestroy(plan->graph));
         plan->exec = nullptr;
         plan->graph = nullptr;
     }
-    return best;
+    return plan->mode;
 }
 
 __attribute__((visibility("default")))
@@ -678,13 +745,16 @@ void bli_amd_gemm_batch_plan_set_mode(void* handle, int mode) {
     auto* plan = (BatchPlan*)handle;
     std::lock_guard<std::mutex> lock(api_mutex());
     if (mode < 0 || mode >= NMODE) abort_with(BLIS_INVALID_UKR_ID, "batch plan mode");
-    if (mode != plan->mode && plan->exec) {
+    bool changed = mode != plan->mode;
+    for (int v = 0; v < NVARIANT; ++v) changed = changed || plan->vmode[v] != mode;
+    if (changed && plan->exec) {
         HIP_CHECK(hipGraphExecDestroy(plan->exec));
         HIP_CHECK(hipGraphDestroy(plan->graph));
         plan->exec = nullptr;
         plan->graph = nullptr;
     }
     plan->mode = mode;
+    for (int v = 0; v < NVARIANT; ++v) plan->vmode[v] = mode;
 }
 
 __attribute__((visibility("default")))

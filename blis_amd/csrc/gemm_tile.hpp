@@ -473,6 +473,69 @@ __device__ inline void store_tile(const TileProblem<T>& p,
     }
 }
 
+// Epilogue through LDS, for tiles whose wave rows per lane are too few for
+// full-width vector access to C: the accumulators are written to LDS as a
+// column-major BM x BN tile (column pitch BM + V), then each thread updates
+// whole 16-byte chunks of C columns, so a wave reads and writes entire
+// cache lines. The arithmetic per element is the same epilogue() as
+// store_tile. smem must hold BN * (BM + V) elements; the caller has
+// synchronised after the last use of smem by the k loop.
+template <typename T, class TL>
+__device__ inline void store_tile_lds(const TileProblem<T>& p,
+                                      const typename Mma<T>::acc_t (&acc)[TL::TM][TL::TN],
+                                      int64_t m0, int64_t n0, int wm0, int wn0, T* smem) {
+    using vec_t = typename Mma<T>::vec_t;
+    constexpr int V = TL::V;
+    constexpr int TM = TL::TM;
+    constexpr int LDC = TL::BM + V;
+    static_assert(TL::BN * LDC <= 2 * TL::STAGE, "epilogue tile must fit the LDS stages");
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, g = lane >> 4, lc = lane & 15;
+#pragma unroll
+    for (int j = 0; j < TL::TN; ++j) {
+        const int col = wn0 + TL::TN * lc + j;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = wm0 + TM * Mma<T>::acc_row(r, g);
+#pragma unroll
+            for (int i = 0; i < TM; ++i) smem[col * LDC + row + i] = acc[i][j][r];
+        }
+    }
+    __syncthreads();
+    constexpr int CH = TL::BM / V;                 // 16-byte chunks per column
+    const T alpha = p.alpha, beta = p.beta;
+#pragma unroll
+    for (int q0 = 0; q0 < CH * TL::BN; q0 += TL::NT) {
+        const int q = q0 + tid;
+        if ((CH * TL::BN) % TL::NT != 0 && q >= CH * TL::BN) break;
+        const int col = q / CH, ch = q % CH;
+        const int64_t gc = n0 + col, gr = m0 + (int64_t)ch * V;
+        if (gc >= p.n || gr >= p.m) continue;
+        const T* src = smem + col * LDC + ch * V;
+        T* dst = p.c + gr * p.c_si + gc * p.c_sj;
+        if (p.vec_c && gr + V <= p.m) {
+            const vec_t a = *reinterpret_cast<const vec_t*>(src);
+            vec_t o;
+            if (beta == T(0)) {
+#pragma unroll
+                for (int e = 0; e < V; ++e) o[e] = alpha * a[e];
+            } else {
+                const vec_t old = *reinterpret_cast<const vec_t*>(dst);
+#pragma unroll
+                for (int e = 0; e < V; ++e) o[e] = epilogue(alpha, a[e], beta, old[e]);
+            }
+            *reinterpret_cast<vec_t*>(dst) = o;
+        } else {
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                if (gr + e >= p.m) break;
+                T* d = dst + e * p.c_si;
+                *d = (beta == T(0)) ? alpha * src[e] : epilogue(alpha, src[e], beta, *d);
+            }
+        }
+    }
+}
+
 // Compute one BM x BN tile (tile coordinates tm, tn) of one problem. smem
 // must hold 2*TL::STAGE elements. The caller synchronises before reusing smem.
 // FAST: the tile lies inside the matrix and A, B (and C) allow 16-byte access,
@@ -486,7 +549,7 @@ __device__ inline void store_tile(const TileProblem<T>& p,
 // loads wherever a whole wave is inside the matrix (fewer instructions per
 // slab: the batch kernels, where almost every tile is an edge tile).
 // The loaded values, and so the arithmetic, are the same in every variant.
-template <typename T, class TL, bool A_K, bool B_K, bool FAST, int PF, bool EDGE>
+template <typename T, class TL, bool A_K, bool B_K, bool FAST, int PF, bool EDGE, bool LDS_EPI>
 __device__ inline void gemm_tile_impl(const TileProblem<T>& p, int64_t m0, int64_t n0, T* smem) {
     static_assert(PF == 1 || PF == 2, "prefetch depth");
     using acc_t = typename Mma<T>::acc_t;
@@ -595,18 +658,23 @@ __device__ inline void gemm_tile_impl(const TileProblem<T>& p, int64_t m0, int64
         }
     }
 
-    if (FAST && p.vec_c)
+    if constexpr (LDS_EPI)
+        store_tile_lds<T, TL>(p, acc, m0, n0, wm0, wn0, smem);
+    else if (FAST && p.vec_c)
         store_tile<T, TL, true>(p, acc, m0 + wm0, n0 + wn0, lane);
     else
         store_tile<T, TL, false>(p, acc, m0 + wm0, n0 + wn0, lane);
 }
 
-template <typename T, class TL, bool A_K, bool B_K, int PF = 1, bool EDGE = false>
+// LDS_EPI: write C through store_tile_lds (the batch kernels) instead of
+// straight from the accumulators.
+template <typename T, class TL, bool A_K, bool B_K, int PF = 1, bool EDGE = false,
+          bool LDS_EPI = false>
 __device__ inline void gemm_tile(const TileProblem<T>& p, int64_t tm, int64_t tn, T* smem) {
     const int64_t m0 = tm * TL::BM, n0 = tn * TL::BN;
     const bool fast = p.vec_a && p.vec_b && m0 + TL::BM <= p.m && n0 + TL::BN <= p.n;
-    if (fast) gemm_tile_impl<T, TL, A_K, B_K, true, PF, EDGE>(p, m0, n0, smem);
-    else gemm_tile_impl<T, TL, A_K, B_K, false, PF, EDGE>(p, m0, n0, smem);
+    if (fast) gemm_tile_impl<T, TL, A_K, B_K, true, PF, EDGE, LDS_EPI>(p, m0, n0, smem);
+    else gemm_tile_impl<T, TL, A_K, B_K, false, PF, EDGE, LDS_EPI>(p, m0, n0, smem);
 }
 
 } // namespace dev

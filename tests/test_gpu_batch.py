@@ -213,6 +213,9 @@ def test_batch_trivial_problems(blis, dtype):
             assert err <= 4 * max(p["k"], 1) * u, (i, err)
 
 
+NMODE = 7   # batch kernel modes
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_batch_modes_and_autotune(blis, dtype):
     """Every kernel mode gives bit-identical results; the autotuner leaves C
@@ -228,14 +231,14 @@ def test_batch_modes_and_autotune(blis, dtype):
     plan = getattr(blis, f"bli_amd_{ch}gemm_batch_plan_create")(*args)
     try:
         results = []
-        for mode in range(5):
+        for mode in range(NMODE):
             blis.bli_amd_gemm_batch_plan_set_mode(plan, mode)
             assert blis.bli_amd_gemm_batch_plan_get_mode(plan) == mode
             for p in probs:
                 p["c"].dbuf.fill_(7.0)
             blis.bli_amd_gemm_batch_plan_execute(plan, mode % 2)
             results.append([p["c"].device_view().copy() for p in probs])
-        for mode in range(1, 5):
+        for mode in range(1, NMODE):
             for r0, r in zip(results[0], results[mode]):
                 np.testing.assert_array_equal(r, r0)
         # the autotuner restores C
@@ -243,7 +246,7 @@ def test_batch_modes_and_autotune(blis, dtype):
             p["c"].dbuf.fill_(3.0)
         before = [p["c"].device_view().copy() for p in probs]
         tuned = blis.bli_amd_gemm_batch_plan_autotune(plan, 2)
-        assert 0 <= tuned < 5 and blis.bli_amd_gemm_batch_plan_get_mode(plan) == tuned
+        assert 0 <= tuned < NMODE and blis.bli_amd_gemm_batch_plan_get_mode(plan) == tuned
         for p, b in zip(probs, before):
             np.testing.assert_array_equal(p["c"].device_view(), b)
         blis.bli_amd_gemm_batch_plan_execute(plan, 1)
