@@ -31,6 +31,9 @@ constexpr int BR = 64;          // rows per register block (one per lane)
 constexpr int P = 4;            // register blocks per leaf
 constexpr int NB = BR * P;      // leaf rows
 constexpr int WPB = 4;          // waves per workgroup
+// Right-hand sides from which the inverted-diagonal path pays: inverting a
+// block costs about as much as solving NB right-hand sides by substitution.
+constexpr int64_t TRSM_INV_MIN_N = 512;
 
 __device__ inline double lane_bcast(double v, int l) {
     const long long b = __double_as_longlong(v);
@@ -52,13 +55,19 @@ __device__ inline float lane_bcast(float v, int l) {
 // then solved by 64 broadcast steps: the solved row i, scaled by the
 // reciprocal of the diagonal, is read from lane i and every other lane
 // subtracts its L element times it.
+// Several independent leaves in one launch: leaf y = blockIdx.y starts at
+// a + y*a_step, b + y*b_step, and the last one has nrows_last rows.
 template <typename T, bool LOWER, int CPW>
 __global__ __launch_bounds__(64 * WPB) void trsm_leaf_kernel(const T* __restrict__ a, int64_t ars,
                                                              int64_t acs, T* __restrict__ b,
                                                              int64_t brs, int64_t bcs,
                                                              int nrows, int64_t n, T alpha,
-                                                             int unit_diag) {
+                                                             int unit_diag, int64_t a_step,
+                                                             int64_t b_step, int nrows_last) {
     __shared__ T xs[WPB][P][CPW][BR];
+    a += (int64_t)blockIdx.y * a_step;
+    b += (int64_t)blockIdx.y * b_step;
+    if (blockIdx.y == gridDim.y - 1) nrows = nrows_last;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int64_t col0 = ((int64_t)blockIdx.x * WPB + wave) * CPW;
@@ -132,6 +141,62 @@ __global__ __launch_bounds__(64 * WPB) void trsm_leaf_kernel(const T* __restrict
         }
 }
 
+// Which diagonal blocks may be solved by multiplying with their inverse.
+// Block q (rows q*NB .. q*NB + nb - 1, one workgroup of NB threads, thread
+// t on row q*NB + t) passes when every row is diagonally dominant by a
+// factor of two within the block, |l_ii| > 2 * sum_{j != i} |l_ij|, and
+// the diagonal magnitudes stay within a factor of 16 of each other. Then
+// L = D (I + D^-1 N) with ||D^-1 N||_inf < 1/2, so ||inv(L)||_inf <=
+// 2 / min|l_ii| and cond_inf(L) <= 3 * max|l_ii| / min|l_ii| <= 48: the
+// product with the FP64 (FP32) inverse keeps the residual of substitution
+// to within that factor.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(NB) void trsm_diag_check_kernel(const T* __restrict__ a, int64_t ars,
+                                                            int64_t acs, int64_t m,
+                                                            int unit_diag, int* __restrict__ ok) {
+    __shared__ int bad;
+    __shared__ float dmax, dmin;
+    const int t = threadIdx.x;
+    const int64_t q0 = (int64_t)blockIdx.x * NB;
+    const int64_t nb = (m - q0 < NB) ? m - q0 : NB;
+    if (t == 0) { bad = 0; dmax = 0.f; dmin = 3.0e38f; }
+    __syncthreads();
+    if (t < nb) {
+        const int64_t r = q0 + t;
+        double off = 0.0;
+        const int64_t j0 = LOWER ? q0 : r + 1, j1 = LOWER ? r : q0 + nb;
+        for (int64_t j = j0; j < j1; ++j) off += fabs((double)a[r * ars + j * acs]);
+        const double d = unit_diag ? 1.0 : fabs((double)a[r * ars + r * acs]);
+        if (!(d > 2.0 * off) || !(d < 1.0e300)) atomicOr(&bad, 1);
+        atomicMax(&dmax, (float)d);
+        atomicMin(&dmin, (float)d);
+    }
+    __syncthreads();
+    if (t == 0) ok[blockIdx.x] = (!bad && dmax <= 16.0f * dmin) ? 1 : 0;
+}
+
+// nblk identity blocks of NB x NB, column-major with leading dimension NB.
+template <typename T>
+__global__ void trsm_identity_kernel(T* __restrict__ w, int64_t total) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e % NB, j = (e / NB) % NB;
+        w[e] = (i == j) ? T(1) : T(0);
+    }
+}
+
+// dst(i, j) = src[i + j*lds] for i < rows, j < cols.
+template <typename T>
+__global__ void trsm_copy_kernel(const T* __restrict__ src, int64_t lds, T* __restrict__ dst,
+                                 int64_t drs, int64_t dcs, int64_t rows, int64_t cols) {
+    const int64_t total = rows * cols;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e % rows, j = e / rows;
+        dst[i * drs + j * dcs] = src[i + j * lds];
+    }
+}
+
 struct Solve {
     Dt dt;
     bool lower;
@@ -140,28 +205,75 @@ struct Solve {
     View b;          // B'(i,j)    = b.ptr[i*b.rs + j*b.cs]
     int64_t n;
     hipStream_t st;
+    // Inverted diagonal blocks (NB x NB each, column-major, ld NB) and the
+    // per-block flag that allows their use; null when not prepared.
+    const void* linv = nullptr;
+    const int* inv_ok = nullptr;
+    void* wtmp = nullptr;   // NB x n staging for a leaf solved by product
 };
 
 inline void* offset(const View& v, int64_t i, int64_t j, size_t es) {
     return (char*)v.ptr + (i * v.rs + j * v.cs) * (int64_t)es;
 }
 
+// Launch `count` leaves of the same shape: leaf y solves the nb-row (the
+// last one nb_last-row) triangle at a + y*a_step for the n columns at
+// b + y*b_step with strides (brs, bcs).
+template <typename T, int CPW>
+void leaf_launch(bool lower, bool unit, const void* a, int64_t ars, int64_t acs, void* b,
+                 int64_t brs, int64_t bcs, int64_t nb, int64_t n, double alpha, int count,
+                 int64_t a_step, int64_t b_step, int64_t nb_last, hipStream_t st) {
+    const int64_t waves = (n + CPW - 1) / CPW;
+    const dim3 grid((unsigned)((waves + WPB - 1) / WPB), (unsigned)count), block(64 * WPB);
+    if (lower)
+        hipLaunchKernelGGL((trsm_leaf_kernel<T, true, CPW>), grid, block, 0, st, (const T*)a, ars,
+                           acs, (T*)b, brs, bcs, (int)nb, n, (T)alpha, (int)unit, a_step, b_step,
+                           (int)nb_last);
+    else
+        hipLaunchKernelGGL((trsm_leaf_kernel<T, false, CPW>), grid, block, 0, st, (const T*)a,
+                           ars, acs, (T*)b, brs, bcs, (int)nb, n, (T)alpha, (int)unit, a_step,
+                           b_step, (int)nb_last);
+    HIP_CHECK(hipGetLastError());
+}
+
 template <typename T, int CPW>
 void leaf_t(const Solve& s, const void* a, void* b, int64_t nb, double alpha) {
-    const int64_t waves = (s.n + CPW - 1) / CPW;
-    const dim3 grid((unsigned)((waves + WPB - 1) / WPB)), block(64 * WPB);
-    if (s.lower)
-        hipLaunchKernelGGL((trsm_leaf_kernel<T, true, CPW>), grid, block, 0, s.st, (const T*)a,
-                           s.a.rs, s.a.cs, (T*)b, s.b.rs, s.b.cs, (int)nb, s.n, (T)alpha,
-                           (int)s.unit);
+    leaf_launch<T, CPW>(s.lower, s.unit, a, s.a.rs, s.a.cs, b, s.b.rs, s.b.cs, nb, s.n, alpha, 1,
+                        0, 0, nb, s.st);
+}
+
+// A leaf whose diagonal block passed the check: X = alpha * inv(L_qq) * B_q
+// as one GEMM into staging, then copied over B_q.
+void leaf_by_product(const Solve& s, int64_t r0, int64_t nb, double alpha) {
+    const size_t es = dt_size(s.dt);
+    const int64_t q = r0 / NB;
+    GemmDesc d;
+    d.dt = s.dt;
+    d.m = nb; d.n = s.n; d.k = nb;
+    d.a = { (char*)s.linv + (size_t)q * NB * NB * es, 1, NB };
+    d.b = { offset(s.b, r0, 0, es), s.b.rs, s.b.cs };
+    d.c = { s.wtmp, 1, nb };
+    d.alpha = alpha;
+    d.beta = 0.0;
+    gemm_run(d, nullptr, s.st);
+    const int64_t total = nb * s.n;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 8192));
+    if (s.dt == Dt::F64)
+        hipLaunchKernelGGL(trsm_copy_kernel<double>, dim3(g), dim3(256), 0, s.st,
+                           (const double*)s.wtmp, nb, (double*)offset(s.b, r0, 0, es), s.b.rs,
+                           s.b.cs, nb, s.n);
     else
-        hipLaunchKernelGGL((trsm_leaf_kernel<T, false, CPW>), grid, block, 0, s.st, (const T*)a,
-                           s.a.rs, s.a.cs, (T*)b, s.b.rs, s.b.cs, (int)nb, s.n, (T)alpha,
-                           (int)s.unit);
+        hipLaunchKernelGGL(trsm_copy_kernel<float>, dim3(g), dim3(256), 0, s.st,
+                           (const float*)s.wtmp, nb, (float*)offset(s.b, r0, 0, es), s.b.rs,
+                           s.b.cs, nb, s.n);
     HIP_CHECK(hipGetLastError());
 }
 
 void leaf(const Solve& s, int64_t r0, int64_t nb, double alpha) {
+    if (s.linv && r0 % NB == 0 && s.inv_ok[r0 / NB]) {
+        leaf_by_product(s, r0, nb, alpha);
+        return;
+    }
     const size_t es = dt_size(s.dt);
     void* a = offset(s.a, r0, r0, es);
     void* b = offset(s.b, r0, 0, es);
@@ -233,6 +345,62 @@ void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag, int64
         s.b = { b.ptr, b.cs, b.rs };
         s.n = m;
         rows = n;
+    }
+    // Many leaves over many columns: check the diagonal blocks, invert them
+    // all in one launch, and solve the blocks that passed by GEMM products.
+    std::vector<int> ok;
+    if (rows > NB && s.n >= TRSM_INV_MIN_N) {
+        const size_t es = dt_size(dt);
+        const int nblk = (int)((rows + NB - 1) / NB);
+        const int64_t last = rows - (int64_t)(nblk - 1) * NB;
+        int* d_ok = (int*)workspace(sizeof(int) * nblk, 15);
+        if (dt == Dt::F64) {
+            if (s.lower)
+                hipLaunchKernelGGL((trsm_diag_check_kernel<double, true>), dim3(nblk), dim3(NB), 0,
+                                   st, (const double*)s.a.ptr, s.a.rs, s.a.cs, rows, (int)s.unit,
+                                   d_ok);
+            else
+                hipLaunchKernelGGL((trsm_diag_check_kernel<double, false>), dim3(nblk), dim3(NB),
+                                   0, st, (const double*)s.a.ptr, s.a.rs, s.a.cs, rows,
+                                   (int)s.unit, d_ok);
+        } else {
+            if (s.lower)
+                hipLaunchKernelGGL((trsm_diag_check_kernel<float, true>), dim3(nblk), dim3(NB), 0,
+                                   st, (const float*)s.a.ptr, s.a.rs, s.a.cs, rows, (int)s.unit,
+                                   d_ok);
+            else
+                hipLaunchKernelGGL((trsm_diag_check_kernel<float, false>), dim3(nblk), dim3(NB), 0,
+                                   st, (const float*)s.a.ptr, s.a.rs, s.a.cs, rows, (int)s.unit,
+                                   d_ok);
+        }
+        HIP_CHECK(hipGetLastError());
+        ok.assign(nblk, 0);
+        HIP_CHECK(hipMemcpyAsync(ok.data(), d_ok, sizeof(int) * nblk, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        bool any = false;
+        for (int v : ok) any = any || v;
+        if (any) {
+            const int64_t total = (int64_t)nblk * NB * NB;
+            void* linv = workspace((size_t)total * es, 13);
+            const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 8192);
+            const int64_t a_step = NB * (s.a.rs + s.a.cs);
+            if (dt == Dt::F64) {
+                hipLaunchKernelGGL(trsm_identity_kernel<double>, dim3(g), dim3(256), 0, st,
+                                   (double*)linv, total);
+                HIP_CHECK(hipGetLastError());
+                leaf_launch<double, 2>(s.lower, s.unit, s.a.ptr, s.a.rs, s.a.cs, linv, 1, NB, NB,
+                                       NB, 1.0, nblk, a_step, (int64_t)NB * NB, last, st);
+            } else {
+                hipLaunchKernelGGL(trsm_identity_kernel<float>, dim3(g), dim3(256), 0, st,
+                                   (float*)linv, total);
+                HIP_CHECK(hipGetLastError());
+                leaf_launch<float, 2>(s.lower, s.unit, s.a.ptr, s.a.rs, s.a.cs, linv, 1, NB, NB,
+                                      NB, 1.0, nblk, a_step, (int64_t)NB * NB, last, st);
+            }
+            s.linv = linv;
+            s.inv_ok = ok.data();
+            s.wtmp = workspace((size_t)NB * s.n * es, 14);
+        }
     }
     solve(s, 0, rows, alpha);
 }

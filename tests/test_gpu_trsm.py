@@ -118,3 +118,49 @@ def test_trsm_config2_residual(blis, dtype):
     resid = np.linalg.norm(af @ x - b0) / (np.linalg.norm(af) * np.linalg.norm(x) * n * unit_roundoff(dtype))
     say(f"trsm config2 scaled residual {resid:.3e}")
     assert resid <= 8.0
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("side", [0, 1])
+@pytest.mark.parametrize("uplo", [LOWER, UPPER])
+@pytest.mark.parametrize("ta", ["n", "t"])
+@pytest.mark.parametrize("diag", [0, 0x100])
+def test_trsm_many_leaves(blis, oracle, dtype, side, uplo, ta, diag):
+    """Solves with several diagonal blocks and many right-hand sides, where
+    diagonal blocks that pass the dominance check are solved by products with
+    their inverses: a dominant triangle (every block takes the product path)
+    and one whose rows are not dominant (every block falls back to
+    substitution), ragged in the last block. Same bar as the small cases."""
+    u = unit_roundoff(dtype)
+    m, n = (700, 530) if side == 0 else (530, 700)
+    mn = m if side == 0 else n
+    for kind in ("dominant", "not dominant"):
+        say(f"trsm many leaves {kind} {np.dtype(dtype).name} side={side} uplo={uplo:#x} {ta} "
+            f"diag={diag:#x} m={m} n={n}")
+        if kind == "dominant":
+            a, _ = tri_matrix(mn, uplo, diag == 0x100, dtype, seed=mn + 5)
+        else:
+            # off-diagonal entries of size 2/sqrt(mn): row sums far above the
+            # diagonal, yet a well-conditioned triangle
+            rng = np.random.default_rng(mn + 6)
+            full = rng.uniform(-1, 1, (mn, mn)) * (2.0 / np.sqrt(mn))
+            full[np.diag_indices(mn)] = 1.0 + rng.uniform(0, 1, mn)
+            if uplo == LOWER:
+                full[np.triu_indices(mn, 1)] = np.nan
+            else:
+                full[np.tril_indices(mn, -1)] = np.nan
+            if diag == 0x100:
+                full[np.diag_indices(mn)] = np.nan
+            a = Mat(mn, mn, dtype=dtype, seed=1)
+            a.set(full.astype(dtype))
+        b = Mat(m, n, dtype=dtype, seed=m + 3 * n)
+        b0 = b.view().copy()
+        a.to_device()
+        b.to_device()
+        run_trsm(blis, dtype, side, uplo, ta, diag, m, n, 2.0, a, b)
+        got = b.device_view()
+        ref = b0.copy(order="F")
+        oracle.trsm(side, uplo, ta, diag, 2.0, a.view().copy(order="F"), ref)
+        err = np.linalg.norm((got - ref).astype(np.float64)) / np.linalg.norm(ref.astype(np.float64))
+        assert np.isfinite(got).all()
+        assert err <= 8 * max(m, n) * u, (kind, err)
