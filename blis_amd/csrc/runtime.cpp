@@ -28,8 +28,8 @@ struct State {
     bool user_stream = false;
     bool sync = true;
     std::mutex ws_mutex;
-    void* ws[16] = {};
-    size_t ws_bytes[16] = {};
+    void* ws[WS_SLOTS] = {};
+    size_t ws_bytes[WS_SLOTS] = {};
     std::mutex api;
 };
 
@@ -85,13 +85,17 @@ int device_id() {
     return S().device;
 }
 
+bool synchronous() {
+    return S().sync != 0;
+}
+
 void sync_if_needed() {
     if (S().sync) HIP_CHECK(hipStreamSynchronize(S().stream));
 }
 
 void* workspace(size_t bytes, int slot) {
     State& s = S();
-    if (slot < 0 || slot >= 16) abort_with(BLIS_INVALID_UKR_ID, "workspace slot");
+    if (slot < 0 || slot >= WS_SLOTS) abort_with(BLIS_INVALID_UKR_ID, "workspace slot");
     std::lock_guard<std::mutex> g(s.ws_mutex);
     if (bytes == 0) bytes = 256;
     if (s.ws_bytes[slot] < bytes) {

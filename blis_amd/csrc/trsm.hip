@@ -163,9 +163,17 @@ __global__ __launch_bounds__(NB) void trsm_diag_check_kernel(const T* __restrict
     __syncthreads();
     if (t < nb) {
         const int64_t r = q0 + t;
-        double off = 0.0;
+        // Eight independent partial sums keep eight loads in flight.
+        double part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         const int64_t j0 = LOWER ? q0 : r + 1, j1 = LOWER ? r : q0 + nb;
-        for (int64_t j = j0; j < j1; ++j) off += fabs((double)a[r * ars + j * acs]);
+        int64_t j = j0;
+        for (; j + 8 <= j1; j += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) part[u] += fabs((double)a[r * ars + (j + u) * acs]);
+        }
+        for (; j < j1; ++j) part[0] += fabs((double)a[r * ars + j * acs]);
+        const double off = ((part[0] + part[1]) + (part[2] + part[3])) +
+                           ((part[4] + part[5]) + (part[6] + part[7]));
         const double d = unit_diag ? 1.0 : fabs((double)a[r * ars + r * acs]);
         if (!(d > 2.0 * off) || !(d < 1.0e300)) atomicOr(&bad, 1);
         atomicMax(&dmax, (float)d);
@@ -185,15 +193,31 @@ __global__ void trsm_identity_kernel(T* __restrict__ w, int64_t total) {
     }
 }
 
-// dst(i, j) = src[i + j*lds] for i < rows, j < cols.
+// dst(i, j) = src(i, j) for i < rows, j < cols, both general-strided, through
+// 64 x 64 LDS tiles: reads run along i and writes along j, so a copy between
+// column-major and row-major storage is coalesced on both sides.
 template <typename T>
-__global__ void trsm_copy_kernel(const T* __restrict__ src, int64_t lds, T* __restrict__ dst,
-                                 int64_t drs, int64_t dcs, int64_t rows, int64_t cols) {
-    const int64_t total = rows * cols;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = e % rows, j = e / rows;
-        dst[i * drs + j * dcs] = src[i + j * lds];
+__global__ __launch_bounds__(256) void trsm_copy_kernel(const T* __restrict__ src, int64_t srs,
+                                                        int64_t scs, T* __restrict__ dst,
+                                                        int64_t drs, int64_t dcs, int64_t rows,
+                                                        int64_t cols) {
+    __shared__ T tile[64][65];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int64_t ti = (rows + 63) / 64, tj = (cols + 63) / 64;
+    for (int64_t t = blockIdx.x; t < ti * tj; t += gridDim.x) {
+        const int64_t i0 = (t % ti) * 64, j0 = (t / ti) * 64;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int64_t i = i0 + tx, j = j0 + ty + 4 * q;
+            if (i < rows && j < cols) tile[tx][ty + 4 * q] = src[i * srs + j * scs];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int64_t i = i0 + ty + 4 * q, j = j0 + tx;
+            if (i < rows && j < cols) dst[i * drs + j * dcs] = tile[ty + 4 * q][tx];
+        }
+        __syncthreads();
     }
 }
 
@@ -209,7 +233,13 @@ struct Solve {
     // per-block flag that allows their use; null when not prepared.
     const void* linv = nullptr;
     const int* inv_ok = nullptr;
-    void* wtmp = nullptr;   // NB x n staging for a leaf solved by product
+    void* wtmp = nullptr;   // NB x n staging for a product leaf solved in place
+    // Where the updates modify the right-hand side and where the solved rows
+    // live. Both are b for the in-place solve; when every block is solved by
+    // product they are separate buffers and B is written once at the end.
+    View rhs;
+    View x;
+    bool separate = false;
 };
 
 inline void* offset(const View& v, int64_t i, int64_t j, size_t es) {
@@ -242,8 +272,9 @@ void leaf_t(const Solve& s, const void* a, void* b, int64_t nb, double alpha) {
                         0, 0, nb, s.st);
 }
 
-// A leaf whose diagonal block passed the check: X = alpha * inv(L_qq) * B_q
-// as one GEMM into staging, then copied over B_q.
+// A leaf whose diagonal block passed the check: X_q = alpha * inv(L_qq) * R_q
+// as one GEMM, straight into x when it is a separate buffer, else into
+// staging and then copied over B_q.
 void leaf_by_product(const Solve& s, int64_t r0, int64_t nb, double alpha) {
     const size_t es = dt_size(s.dt);
     const int64_t q = r0 / NB;
@@ -251,20 +282,25 @@ void leaf_by_product(const Solve& s, int64_t r0, int64_t nb, double alpha) {
     d.dt = s.dt;
     d.m = nb; d.n = s.n; d.k = nb;
     d.a = { (char*)s.linv + (size_t)q * NB * NB * es, 1, NB };
-    d.b = { offset(s.b, r0, 0, es), s.b.rs, s.b.cs };
-    d.c = { s.wtmp, 1, nb };
+    d.b = { offset(s.rhs, r0, 0, es), s.rhs.rs, s.rhs.cs };
     d.alpha = alpha;
     d.beta = 0.0;
+    if (s.separate) {
+        d.c = { offset(s.x, r0, 0, es), s.x.rs, s.x.cs };
+        gemm_run(d, nullptr, s.st);
+        return;
+    }
+    d.c = { s.wtmp, 1, nb };
     gemm_run(d, nullptr, s.st);
-    const int64_t total = nb * s.n;
-    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 8192));
+    const int64_t tiles = ((nb + 63) / 64) * ((s.n + 63) / 64);
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(tiles, 8192));
     if (s.dt == Dt::F64)
         hipLaunchKernelGGL(trsm_copy_kernel<double>, dim3(g), dim3(256), 0, s.st,
-                           (const double*)s.wtmp, nb, (double*)offset(s.b, r0, 0, es), s.b.rs,
+                           (const double*)s.wtmp, 1, nb, (double*)offset(s.b, r0, 0, es), s.b.rs,
                            s.b.cs, nb, s.n);
     else
         hipLaunchKernelGGL(trsm_copy_kernel<float>, dim3(g), dim3(256), 0, s.st,
-                           (const float*)s.wtmp, nb, (float*)offset(s.b, r0, 0, es), s.b.rs,
+                           (const float*)s.wtmp, 1, nb, (float*)offset(s.b, r0, 0, es), s.b.rs,
                            s.b.cs, nb, s.n);
     HIP_CHECK(hipGetLastError());
 }
@@ -283,7 +319,7 @@ void leaf(const Solve& s, int64_t r0, int64_t nb, double alpha) {
     else leaf_t<float, 2>(s, a, b, nb, alpha);
 }
 
-// B'[dst rows] := beta * B'[dst rows] - L_eff[dst rows, src rows] * X[src rows]
+// R[dst rows] := beta * R[dst rows] - L_eff[dst rows, src rows] * X[src rows]
 void update(const Solve& s, int64_t dst0, int64_t dstn, int64_t src0, int64_t srcn,
             double beta) {
     const size_t es = dt_size(s.dt);
@@ -291,8 +327,8 @@ void update(const Solve& s, int64_t dst0, int64_t dstn, int64_t src0, int64_t sr
     d.dt = s.dt;
     d.m = dstn; d.n = s.n; d.k = srcn;
     d.a = { offset(s.a, dst0, src0, es), s.a.rs, s.a.cs };
-    d.b = { offset(s.b, src0, 0, es), s.b.rs, s.b.cs };
-    d.c = { offset(s.b, dst0, 0, es), s.b.rs, s.b.cs };
+    d.b = { offset(s.x, src0, 0, es), s.x.rs, s.x.cs };
+    d.c = { offset(s.rhs, dst0, 0, es), s.rhs.rs, s.rhs.cs };
     d.alpha = -1.0;
     d.beta = beta;
     gemm_run(d, nullptr, s.st);
@@ -346,6 +382,8 @@ void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag, int64
         s.n = m;
         rows = n;
     }
+    s.rhs = s.b;
+    s.x = s.b;
     // Many leaves over many columns: check the diagonal blocks, invert them
     // all in one launch, and solve the blocks that passed by GEMM products.
     std::vector<int> ok;
@@ -354,6 +392,10 @@ void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag, int64
         const int nblk = (int)((rows + NB - 1) / NB);
         const int64_t last = rows - (int64_t)(nblk - 1) * NB;
         int* d_ok = (int*)workspace(sizeof(int) * nblk, 15);
+        const int64_t total = (int64_t)nblk * NB * NB;
+        void* linv = workspace((size_t)total * es, 13);
+        const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 8192);
+        const int64_t a_step = NB * (s.a.rs + s.a.cs);
         if (dt == Dt::F64) {
             if (s.lower)
                 hipLaunchKernelGGL((trsm_diag_check_kernel<double, true>), dim3(nblk), dim3(NB), 0,
@@ -374,33 +416,63 @@ void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag, int64
                                    d_ok);
         }
         HIP_CHECK(hipGetLastError());
+        // The check is read back (one wait) while the GPU inverts the blocks;
+        // the flags land in pinned host memory so the copy is asynchronous.
         ok.assign(nblk, 0);
-        HIP_CHECK(hipMemcpyAsync(ok.data(), d_ok, sizeof(int) * nblk, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipStreamSynchronize(st));
-        bool any = false;
-        for (int v : ok) any = any || v;
-        if (any) {
-            const int64_t total = (int64_t)nblk * NB * NB;
-            void* linv = workspace((size_t)total * es, 13);
-            const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 8192);
-            const int64_t a_step = NB * (s.a.rs + s.a.cs);
-            if (dt == Dt::F64) {
-                hipLaunchKernelGGL(trsm_identity_kernel<double>, dim3(g), dim3(256), 0, st,
-                                   (double*)linv, total);
-                HIP_CHECK(hipGetLastError());
-                leaf_launch<double, 2>(s.lower, s.unit, s.a.ptr, s.a.rs, s.a.cs, linv, 1, NB, NB,
-                                       NB, 1.0, nblk, a_step, (int64_t)NB * NB, last, st);
-            } else {
-                hipLaunchKernelGGL(trsm_identity_kernel<float>, dim3(g), dim3(256), 0, st,
-                                   (float*)linv, total);
-                HIP_CHECK(hipGetLastError());
-                leaf_launch<float, 2>(s.lower, s.unit, s.a.ptr, s.a.rs, s.a.cs, linv, 1, NB, NB,
-                                      NB, 1.0, nblk, a_step, (int64_t)NB * NB, last, st);
-            }
-            s.linv = linv;
-            s.inv_ok = ok.data();
-            s.wtmp = workspace((size_t)NB * s.n * es, 14);
+        static int* pinned = nullptr;
+        constexpr int PINNED_FLAGS = 1 << 16;
+        if (!pinned) HIP_CHECK(hipHostMalloc((void**)&pinned, sizeof(int) * PINNED_FLAGS));
+        int* flags = nblk <= PINNED_FLAGS ? pinned : ok.data();
+        HIP_CHECK(hipMemcpyAsync(flags, d_ok, sizeof(int) * nblk, hipMemcpyDeviceToHost, st));
+        hipEvent_t checked;
+        HIP_CHECK(hipEventCreateWithFlags(&checked, hipEventDisableTiming));
+        HIP_CHECK(hipEventRecord(checked, st));
+        if (dt == Dt::F64) {
+            hipLaunchKernelGGL(trsm_identity_kernel<double>, dim3(g), dim3(256), 0, st,
+                               (double*)linv, total);
+            HIP_CHECK(hipGetLastError());
+            leaf_launch<double, 2>(s.lower, s.unit, s.a.ptr, s.a.rs, s.a.cs, linv, 1, NB, NB, NB,
+                                   1.0, nblk, a_step, (int64_t)NB * NB, last, st);
+        } else {
+            hipLaunchKernelGGL(trsm_identity_kernel<float>, dim3(g), dim3(256), 0, st,
+                               (float*)linv, total);
+            HIP_CHECK(hipGetLastError());
+            leaf_launch<float, 2>(s.lower, s.unit, s.a.ptr, s.a.rs, s.a.cs, linv, 1, NB, NB, NB,
+                                  1.0, nblk, a_step, (int64_t)NB * NB, last, st);
         }
+        HIP_CHECK(hipEventSynchronize(checked));
+        HIP_CHECK(hipEventDestroy(checked));
+        if (flags != ok.data()) std::copy(flags, flags + nblk, ok.begin());
+        s.linv = linv;
+        s.inv_ok = ok.data();
+        auto copy = [&](View src, View dst) {
+            const int64_t tiles = ((rows + 63) / 64) * ((s.n + 63) / 64);
+            const unsigned gc = (unsigned)std::max<int64_t>(1, std::min<int64_t>(tiles, 8192));
+            if (dt == Dt::F64)
+                hipLaunchKernelGGL(trsm_copy_kernel<double>, dim3(gc), dim3(256), 0, st,
+                                   (const double*)src.ptr, src.rs, src.cs, (double*)dst.ptr,
+                                   dst.rs, dst.cs, rows, s.n);
+            else
+                hipLaunchKernelGGL(trsm_copy_kernel<float>, dim3(gc), dim3(256), 0, st,
+                                   (const float*)src.ptr, src.rs, src.cs, (float*)dst.ptr,
+                                   dst.rs, dst.cs, rows, s.n);
+            HIP_CHECK(hipGetLastError());
+        };
+        bool all = true;
+        for (int v : ok) all = all && v;
+        if (all) {
+            // Every block by product: the right-hand side is updated in place
+            // and X is built in a separate column-major buffer, copied over B
+            // once at the end (no per-leaf staging copies).
+            s.x = { workspace((size_t)rows * s.n * es, 16), 1, rows };
+            s.separate = true;
+            solve(s, 0, rows, alpha);
+            copy(s.x, s.b);
+            return;
+        }
+        // Some blocks by substitution: in place, product leaves through
+        // staging.
+        s.wtmp = workspace((size_t)NB * s.n * es, 14);
     }
     solve(s, 0, rows, alpha);
 }

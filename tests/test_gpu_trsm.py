@@ -164,3 +164,37 @@ def test_trsm_many_leaves(blis, oracle, dtype, side, uplo, ta, diag):
         err = np.linalg.norm((got - ref).astype(np.float64)) / np.linalg.norm(ref.astype(np.float64))
         assert np.isfinite(got).all()
         assert err <= 8 * max(m, n) * u, (kind, err)
+
+
+def test_trsm_many_leaves_async(blis, oracle):
+    """The same solves with the library in asynchronous mode (the solve waits
+    once, for the check of the diagonal blocks)."""
+    import torch
+    m, n = 700, 530
+    for kind, scale in (("dominant", None), ("not dominant", 2.0)):
+        say(f"trsm many leaves async {kind}")
+        if scale is None:
+            a, _ = tri_matrix(m, LOWER, False, np.float64, seed=m + 5)
+        else:
+            rng = np.random.default_rng(m + 6)
+            full = rng.uniform(-1, 1, (m, m)) * (scale / np.sqrt(m))
+            full[np.diag_indices(m)] = 1.0 + rng.uniform(0, 1, m)
+            full[np.triu_indices(m, 1)] = np.nan
+            a = Mat(m, m, seed=1)
+            a.set(full)
+        b = Mat(m, n, seed=m + 3 * n)
+        b0 = b.view().copy()
+        a.to_device()
+        b.to_device()
+        blis.bli_amd_set_stream(None, 0)
+        try:
+            run_trsm(blis, np.float64, 0, LOWER, "n", 0, m, n, 2.0, a, b)
+            blis.bli_amd_synchronize()
+        finally:
+            blis.bli_amd_set_stream(None, 1)
+        torch.cuda.synchronize()
+        got = b.device_view()
+        ref = b0.copy(order="F")
+        oracle.trsm(0, LOWER, "n", 0, 2.0, a.view().copy(order="F"), ref)
+        err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        assert err <= 8 * max(m, n) * unit_roundoff(np.float64), (kind, err)
