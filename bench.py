@@ -135,6 +135,29 @@ def cpu_baseline():
                       f"the ic loop) on a {s}^3 NN sample, best of 2, {threads} threads"}
 
 
+def trsm_ms(bench: Bench, m=4096, n=1024):
+    """DTRSM L/L/N/N: A lower with diagonal 4096 + U[0,1] and off-diagonal
+    U[-1,1], B U[-1,1], alpha = 1. B is restored before every call; the
+    restore copy's own time is subtracted."""
+    torch = bench.torch
+    # Row-major upper triangle == column-major lower triangle.
+    a = torch.triu(bench.uniform(m * m, torch.float64).view(m, m)).contiguous()
+    a.diagonal().add_(4096.0)
+    b0 = bench.uniform(m * n, torch.float64)
+    b = b0.clone()
+    al = C.c_double(1.0)
+
+    def trsm():
+        b.copy_(b0)
+        bench.L.bli_dtrsm(0, 0xC0, 0, 0, m, n, C.byref(al), a.data_ptr(), 1, m, b.data_ptr(), 1,
+                          m)
+    ms_copy = bench.kernel_ms(lambda: b.copy_(b0), 10)
+    ms = bench.kernel_ms(trsm, 10) - ms_copy
+    del a, b, b0
+    torch.cuda.empty_cache()
+    return ms
+
+
 def extras(bench: Bench):
     torch = bench.torch
     out = {}
@@ -164,21 +187,8 @@ def extras(bench: Bench):
 
     # DTRSM left/lower/no-trans/non-unit, m = 4096, n = 1024.
     m, n = 4096, 1024
-    # Row-major upper triangle == column-major lower triangle.
-    a = torch.triu(bench.uniform(m * m, torch.float64).view(m, m)).contiguous()
-    a.diagonal().add_(4096.0)
-    b0 = bench.uniform(m * n, torch.float64)
-    b = b0.clone()
-    al = C.c_double(1.0)
-
-    def trsm():
-        b.copy_(b0)
-        L.bli_dtrsm(0, 0xC0, 0, 0, m, n, C.byref(al), a.data_ptr(), 1, m, b.data_ptr(), 1, m)
-    ms_copy = bench.kernel_ms(lambda: b.copy_(b0), 10)
-    ms = bench.kernel_ms(trsm, 10) - ms_copy
-    rec("dtrsm_llnn_4096x1024", ms, float(m) * m * n, "TFLOP/s", PEAK_TFLOPS["f64"], "mfma")
-    del a, b, b0
-    torch.cuda.empty_cache()
+    rec("dtrsm_llnn_4096x1024", trsm_ms(bench), float(m) * m * n, "TFLOP/s", PEAK_TFLOPS["f64"],
+        "mfma")
 
     # DGEMV 16384^2 (N and T), column-major.
     m = n = 16384
@@ -384,6 +394,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-split", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--only-batch", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--only-trsm", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one 1 GiB device copy, then two headline DGEMM calls (for "
                          "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE calibration)")
@@ -397,6 +408,12 @@ def main():
     bench = Bench()
     if args.only_split:
         print(json.dumps(split_bench(bench, log)), flush=True)
+        return
+    if args.only_trsm:
+        ms = trsm_ms(bench)
+        print(json.dumps({"dtrsm_llnn_4096x1024": {"ms": round(ms, 4),
+                                                   "TFLOP/s": round(4096.0 ** 2 * 1024 / ms / 1e9, 3)}}),
+              flush=True)
         return
     if args.only_batch:
         res = {}
