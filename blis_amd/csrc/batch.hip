@@ -78,19 +78,21 @@ struct XcdSegs {
 //   2: two slabs in flight, precomputed edge addressing;
 //   3: two slabs in flight, guarded element loads;
 //   4: wave tiles (gemm_batch_wave_kernel);
-//   5: as 0, with registers capped for four workgroups per CU and C written
-//      through LDS in whole cache lines;
+//   5: as 0, with registers capped for four workgroups per CU, the C tile
+//      pulled into L2 one slab ahead and C written through LDS in whole
+//      cache lines;
 //   6: as 0, with registers capped for four workgroups per CU.
 // Modes other than 4 can differ between the variants of one plan.
 constexpr int NMODE = 7;
 constexpr int WAVE_MODE = 4;
 // Mode used until the autotuner runs: the one measured fastest on uneven
-// batches of 10,000 problems (dims log-uniform in [8, 512]) for each type.
-constexpr int default_mode(Dt dt) { return dt == Dt::F64 ? 6 : 5; }
+// batches of 10,000 problems (dims log-uniform in [8, 512]), for both types.
+constexpr int default_mode(Dt) { return 5; }
 constexpr int mode_pf(int mode) { return (mode == 2 || mode == 3) ? 2 : 1; }
 constexpr bool mode_edge(int mode) { return mode == 1 || mode == 2; }
 constexpr int mode_waves(int mode) { return (mode == 5 || mode == 6) ? 4 : 1; }
 constexpr bool mode_lds_epi(int mode) { return mode == 5; }
+constexpr bool mode_c_pf(int mode) { return mode == 5; }
 
 template <typename T, class TL, bool A_K, bool B_K, int MODE>
 __global__ __launch_bounds__(TL::NT)
@@ -98,6 +100,7 @@ __attribute__((amdgpu_waves_per_eu(mode_waves(MODE)))) void gemm_batch_kernel(co
                                                             const WorkItem* __restrict__ items,
                                                             XcdSegs seg) {
     __shared__ T smem[2 * TL::STAGE];
+    __shared__ int sink[mode_c_pf(MODE) ? TL::NT : 1];
     const int x = blockIdx.x % NXCD;
     const int slot = blockIdx.x / NXCD, nslot = gridDim.x / NXCD;   // gridDim.x % NXCD == 0
     const int end = seg.off[x + 1];
@@ -126,8 +129,8 @@ __attribute__((amdgpu_waves_per_eu(mode_waves(MODE)))) void gemm_batch_kernel(co
         p.vec_b = (d.flags & F_VEC_B) != 0;
         p.vec_c = (d.flags & F_VEC_C) != 0;
         const int64_t tm = w.tile % d.tiles_m, tn = w.tile / d.tiles_m;
-        gemm_tile<T, TL, A_K, B_K, mode_pf(MODE), mode_edge(MODE), mode_lds_epi(MODE)>(p, tm, tn,
-                                                                                  smem);
+        gemm_tile<T, TL, A_K, B_K, mode_pf(MODE), mode_edge(MODE), mode_lds_epi(MODE),
+                  mode_c_pf(MODE)>(p, tm, tn, smem, sink);
         __syncthreads();
         if (!has_next) break;
         it = nit;

@@ -549,8 +549,29 @@ __device__ inline void store_tile_lds(const TileProblem<T>& p,
 // loads wherever a whole wave is inside the matrix (fewer instructions per
 // slab: the batch kernels, where almost every tile is an edge tile).
 // The loaded values, and so the arithmetic, are the same in every variant.
-template <typename T, class TL, bool A_K, bool B_K, bool FAST, int PF, bool EDGE, bool LDS_EPI>
-__device__ inline void gemm_tile_impl(const TileProblem<T>& p, int64_t m0, int64_t n0, T* smem) {
+// Pull the C tile's cache lines into L2 ahead of the epilogue (beta != 0):
+// one 4-byte global->LDS load per 128-byte line of each tile column, into a
+// scratch LDS area of NT words that nothing reads. Addresses are clamped
+// into the matrix. C must be column-stored (c_si == 1).
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+template <typename T, class TL>
+__device__ inline void prefetch_c(const TileProblem<T>& p, int64_t m0, int64_t n0, int* sink) {
+    constexpr int LPC = (TL::BM * (int)sizeof(T) + 127) / 128;   // lines per column
+    constexpr int ROWS_PER_LINE = 128 / (int)sizeof(T);
+    const int tid = threadIdx.x;
+    if (p.beta == T(0) || p.c_si != 1) return;
+    const int col = tid / LPC, line = tid % LPC;
+    if (col >= TL::BN) return;
+    const int64_t r = min(m0 + (int64_t)line * ROWS_PER_LINE, p.m - 1);
+    const int64_t c = min(n0 + (int64_t)col, p.n - 1);
+    int* dst = sink + (tid & ~63);
+    __builtin_amdgcn_global_load_lds((const void*)(p.c + r + c * p.c_sj), (lds_ptr_t)dst, 4, 0, 0);
+}
+
+template <typename T, class TL, bool A_K, bool B_K, bool FAST, int PF, bool EDGE, bool LDS_EPI,
+          bool C_PF = false>
+__device__ inline void gemm_tile_impl(const TileProblem<T>& p, int64_t m0, int64_t n0, T* smem,
+                                      int* sink = nullptr) {
     static_assert(PF == 1 || PF == 2, "prefetch depth");
     using acc_t = typename Mma<T>::acc_t;
     const int tid = threadIdx.x;
@@ -610,11 +631,17 @@ __device__ inline void gemm_tile_impl(const TileProblem<T>& p, int64_t m0, int64
             bl.store(smem + TL::A_ELEMS, tid);
         }
         __syncthreads();
+        if constexpr (C_PF) {
+            if (nk <= 1) prefetch_c<T, TL>(p, m0, n0, sink);
+        }
         for (int64_t s = 0; s < nk; ++s) {
             T* cur = smem + (s & 1) * TL::STAGE;
             T* nxt = smem + ((s + 1) & 1) * TL::STAGE;
             const bool more = (s + 1) < nk;
             if (more) load(al, bl, (s + 1) * TL::BK);
+            if constexpr (C_PF) {
+                if (s + 2 == nk) prefetch_c<T, TL>(p, m0, n0, sink);
+            }
             mma_slab<T, TL>(cur, cur + TL::A_ELEMS, acc, wm0, wn0, lane);
             if (more) {
                 al.store(nxt, tid);
@@ -667,14 +694,20 @@ __device__ inline void gemm_tile_impl(const TileProblem<T>& p, int64_t m0, int64
 }
 
 // LDS_EPI: write C through store_tile_lds (the batch kernels) instead of
-// straight from the accumulators.
+// straight from the accumulators. C_PF: prefetch the C tile into L2 one
+// slab before the epilogue (one-slab prefetch depth only), through `sink`
+// (NT ints of LDS nobody reads).
 template <typename T, class TL, bool A_K, bool B_K, int PF = 1, bool EDGE = false,
-          bool LDS_EPI = false>
-__device__ inline void gemm_tile(const TileProblem<T>& p, int64_t tm, int64_t tn, T* smem) {
+          bool LDS_EPI = false, bool C_PF = false>
+__device__ inline void gemm_tile(const TileProblem<T>& p, int64_t tm, int64_t tn, T* smem,
+                                 int* sink = nullptr) {
+    static_assert(!C_PF || PF == 1, "C prefetch is placed for one slab in flight");
     const int64_t m0 = tm * TL::BM, n0 = tn * TL::BN;
     const bool fast = p.vec_a && p.vec_b && m0 + TL::BM <= p.m && n0 + TL::BN <= p.n;
-    if (fast) gemm_tile_impl<T, TL, A_K, B_K, true, PF, EDGE, LDS_EPI>(p, m0, n0, smem);
-    else gemm_tile_impl<T, TL, A_K, B_K, false, PF, EDGE, LDS_EPI>(p, m0, n0, smem);
+    if (fast)
+        gemm_tile_impl<T, TL, A_K, B_K, true, PF, EDGE, LDS_EPI, C_PF>(p, m0, n0, smem, sink);
+    else
+        gemm_tile_impl<T, TL, A_K, B_K, false, PF, EDGE, LDS_EPI, C_PF>(p, m0, n0, smem, sink);
 }
 
 } // namespace dev
