@@ -451,8 +451,9 @@ int split_choose_slices(Dt dt, double gap_bits) {
 
 // Slice count at which the INT8 emulation stops paying: past it the slice
 // products take longer than native FP64 (FP32) MFMA. The value comes from
-// gfx950 timings of the two 8192^3 kernels.
-int split_max_slices(Dt dt) { return dt == Dt::F64 ? 9 : 6; }
+// gfx950 timings of the two 8192^3 kernels: FP64 at 8 slices (36 products)
+// ties native DGEMM (16.1 ms each), 9 slices would take about 20 ms.
+int split_max_slices(Dt dt) { return dt == Dt::F64 ? 8 : 6; }
 
 static void launch_stats(Dt dt, View v, int64_t rows, int64_t K, int64_t s_row, int64_t s_k,
                          int* emax, long long* esum, int* ecnt, hipStream_t st) {
