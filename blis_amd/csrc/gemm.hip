@@ -280,21 +280,23 @@ int gemm_default_tile(Dt dt, int64_t m, int64_t n, int64_t k) {
     // large-problem tiles are the ones with high wave occupancy (several
     // waves per SIMD hide LDS and barrier latency); small problems use small
     // tiles so that enough workgroups exist to fill the 256 CUs.
-    // Re-measured with exhaustive autotuning at m, n in 4096..16384 and k in
-    // 128..8192: FP64 128x128 tiles of eight 64x32 waves beat the sixteen
-    // 32x32 waves by 4-8% (8192^3 NN 16.09 vs 16.82 ms); FP32 short-k
-    // products prefer 128x64 tiles.
+    // Large shapes re-measured with exhaustive autotuning at m, n in
+    // 4096..16384 and k in 128..8192: FP64 128x128 tiles of eight 64x32
+    // waves beat sixteen 32x32 waves by 4-8% (8192^3 NN 16.09 vs 16.82 ms);
+    // FP32 short-k products prefer 128x64 tiles. Mid-size bands (256 to
+    // 1023 tiles of 128x128) take the tile with the least total time over
+    // the tuning database's shapes in that band.
     auto tiles = [&](int bm, int bn) { return ((m + bm - 1) / bm) * ((n + bn - 1) / bn); };
     if (dt == Dt::F64) {
         if (tiles(128, 128) >= 1024) return 6;   // 128x128 (BK 16), 8 waves of 64x32
-        if (tiles(128, 64) >= 1024) return 7;    // 128x64, 8 waves of 32x32
-        if (tiles(64, 64) >= 512) return 3;
-        return 4;
+        if (tiles(128, 128) >= 256) return 10;   // 64x128 (BK 16), 8 waves of 32x32
+        if (tiles(64, 64) >= 512) return 3;      // 64x64 (BK 16), 4 waves of 32x32
+        return 4;                                // 32x32 (BK 16), 4 waves of 16x16
     }
     if (k <= 256 && tiles(128, 64) >= 512) return 1;   // 128x64 (BK 32), 4 waves of 64x32
     if (tiles(256, 128) >= 512) return 12;       // 256x128 (BK 16), 8 waves of 64x64
-    if (tiles(128, 128) >= 1024) return 8;       // 128x128 (BK 16), 4 waves of 64x64
-    if (tiles(128, 64) >= 512) return 1;
+    if (tiles(128, 128) >= 512) return 9;        // 128x64 (BK 32), 8 waves of 32x32
+    if (tiles(128, 128) >= 256) return 10;       // 64x128 (BK 32), 8 waves of 32x32
     if (tiles(64, 64) >= 512) return 3;
     return 4;
 }
