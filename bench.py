@@ -360,7 +360,7 @@ def batch_bench(bench, ch, key, log, count=10000, seed=42):
     cargs = [C.cast(x, C.c_void_p) for x in args] + [count, C.cast(gs, C.c_void_p)]
     plan = getattr(L, f"bli_amd_{ch}gemm_batch_plan_create")(*cargs)
     modes = {}
-    for mode in range(7):
+    for mode in range(9):
         L.bli_amd_gemm_batch_plan_set_mode(plan, mode)
         modes[mode] = round(bench.kernel_ms(lambda: L.bli_amd_gemm_batch_plan_execute(plan, 1), 10), 4)
     tuned = L.bli_amd_gemm_batch_plan_autotune(plan, 5)

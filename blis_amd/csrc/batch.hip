@@ -51,10 +51,12 @@ struct WorkItem {
 template <typename T> struct BatchTiles;
 template <> struct BatchTiles<double> {
     using Big = TileCfg<double, 64, 64, 16, 32, 32>;
+    using BigHalfK = TileCfg<double, 64, 64, 8, 32, 32>;
     using Small = TileCfg<double, 32, 32, 16, 16, 16>;
 };
 template <> struct BatchTiles<float> {
     using Big = TileCfg<float, 64, 64, 32, 32, 32>;
+    using BigHalfK = TileCfg<float, 64, 64, 16, 32, 32>;
     using Small = TileCfg<float, 32, 32, 32, 16, 16>;
 };
 
@@ -81,18 +83,24 @@ struct XcdSegs {
 //   5: as 0, with registers capped for four workgroups per CU, the C tile
 //      pulled into L2 one slab ahead and C written through LDS in whole
 //      cache lines;
-//   6: as 0, with registers capped for four workgroups per CU.
+//   6: as 0, with registers capped for four workgroups per CU;
+//   7, 8: half the k slab on 64 x 64 tiles (half the LDS and staging
+//      registers), registers capped for five / six workgroups per CU; FP32
+//      also pulls C into L2 and writes it through LDS as in mode 5.
 // Modes other than 4 can differ between the variants of one plan.
-constexpr int NMODE = 7;
+constexpr int NMODE = 9;
 constexpr int WAVE_MODE = 4;
 // Mode used until the autotuner runs: the one measured fastest on uneven
 // batches of 10,000 problems (dims log-uniform in [8, 512]), for both types.
-constexpr int default_mode(Dt) { return 5; }
+constexpr int default_mode(Dt) { return 8; }
 constexpr int mode_pf(int mode) { return (mode == 2 || mode == 3) ? 2 : 1; }
 constexpr bool mode_edge(int mode) { return mode == 1 || mode == 2; }
-constexpr int mode_waves(int mode) { return (mode == 5 || mode == 6) ? 4 : 1; }
-constexpr bool mode_lds_epi(int mode) { return mode == 5; }
-constexpr bool mode_c_pf(int mode) { return mode == 5; }
+constexpr int mode_waves(int mode) { return mode == 7 ? 5 : mode == 8 ? 6 : (mode == 5 || mode == 6) ? 4 : 1; }
+// The C tile fits the half-k stages only for FP32 (64 x 68 floats = two stages).
+template <typename T> constexpr bool mode_lds_epi(int mode) {
+    return mode == 5 || (mode >= 7 && std::is_same<T, float>::value);
+}
+template <typename T> constexpr bool mode_c_pf(int mode) { return mode_lds_epi<T>(mode); }
 
 template <typename T, class TL, bool A_K, bool B_K, int MODE>
 __global__ __launch_bounds__(TL::NT)
@@ -100,7 +108,7 @@ __attribute__((amdgpu_waves_per_eu(mode_waves(MODE)))) void gemm_batch_kernel(co
                                                             const WorkItem* __restrict__ items,
                                                             XcdSegs seg) {
     __shared__ T smem[2 * TL::STAGE];
-    __shared__ int sink[mode_c_pf(MODE) ? TL::NT : 1];
+    __shared__ int sink[mode_c_pf<T>(MODE) ? TL::NT : 1];
     const int x = blockIdx.x % NXCD;
     const int slot = blockIdx.x / NXCD, nslot = gridDim.x / NXCD;   // gridDim.x % NXCD == 0
     const int end = seg.off[x + 1];
@@ -129,8 +137,8 @@ __attribute__((amdgpu_waves_per_eu(mode_waves(MODE)))) void gemm_batch_kernel(co
         p.vec_b = (d.flags & F_VEC_B) != 0;
         p.vec_c = (d.flags & F_VEC_C) != 0;
         const int64_t tm = w.tile % d.tiles_m, tn = w.tile / d.tiles_m;
-        gemm_tile<T, TL, A_K, B_K, mode_pf(MODE), mode_edge(MODE), mode_lds_epi(MODE),
-                  mode_c_pf(MODE)>(p, tm, tn, smem, sink);
+        gemm_tile<T, TL, A_K, B_K, mode_pf(MODE), mode_edge(MODE), mode_lds_epi<T>(MODE),
+                  mode_c_pf<T>(MODE)>(p, tm, tn, smem, sink);
         __syncthreads();
         if (!has_next) break;
         it = nit;
@@ -240,7 +248,8 @@ inline int variant_of(int32_t flags) {
 
 template <typename T, int MODE>
 inline const void* variant_fn(int v) {
-    using Big = typename BatchTiles<T>::Big;
+    using Big = typename std::conditional<(MODE >= 7), typename BatchTiles<T>::BigHalfK,
+                                          typename BatchTiles<T>::Big>::type;
     using Small = typename BatchTiles<T>::Small;
     switch (v) {
     case 0: return (const void*)&gemm_batch_kernel<T, Big, false, false, MODE>;
@@ -277,6 +286,8 @@ inline void variant_info(int mode, int v, const void*& fn, int& threads) {
     case 3: fn = variant_fn<T, 3>(v); return;
     case 5: fn = variant_fn<T, 5>(v); return;
     case 6: fn = variant_fn<T, 6>(v); return;
+    case 7: fn = variant_fn<T, 7>(v); return;
+    case 8: fn = variant_fn<T, 8>(v); return;
     }
     abort_with(BLIS_INVALID_UKR_ID, "batch: mode");
 }

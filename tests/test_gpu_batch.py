@@ -213,7 +213,7 @@ def test_batch_trivial_problems(blis, dtype):
             assert err <= 4 * max(p["k"], 1) * u, (i, err)
 
 
-NMODE = 7   # batch kernel modes
+NMODE = 9   # batch kernel modes
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
