@@ -188,6 +188,7 @@ struct Plan {
     const void* a; int64_t lda; bool a_k;
     const void* b; int64_t ldb; bool b_k;
     void* c; int64_t c_si, c_sj;
+    bool a_packed = false, b_packed = false;   // operand copied into workspace
 };
 
 static void transpose_run(Dt dt, const void* src, int64_t rows, int64_t cols, int64_t lds,
@@ -203,11 +204,15 @@ static void transpose_run(Dt dt, const void* src, int64_t rows, int64_t cols, in
     HIP_CHECK(hipGetLastError());
 }
 
-// A k-contiguous operand is staged into LDS by transposing scalar stores,
-// which measured about 4% slower (FP64 8192^3 NN vs NT) than the
-// rows-contiguous path. For large problems it pays to transpose the operand
-// once (rows x k, reused `reuse` times) and take the faster path; the
-// arithmetic is the same either way.
+// The FP64 128x128 tile of sixteen 32x32 waves (tile 13) stages a
+// k-contiguous operand into LDS by transposing scalar stores, measured
+// about 4% slower (8192^3 NN vs NT) than its rows-contiguous path. For large
+// problems on that tile it pays to transpose the operand once (rows x k,
+// reused `reuse` times) and take the faster path; the arithmetic is the same
+// either way. The default large FP64 tile (6, eight 64x32 waves) stages it
+// within 1.2% of its rows-contiguous path, less than the transpose costs
+// (8192^3 NN: 16.11 ms as is, 16.23 ms transposed), so it takes the operand
+// as it comes.
 static bool transpose_pays(Dt dt, int64_t reuse, int64_t rows, int64_t k) {
     const double es = (double)dt_size(dt);
     const double gemm_s = 2.0 * (double)reuse * rows * k / (dt == Dt::F64 ? 60e12 : 120e12);
@@ -235,29 +240,34 @@ static Plan normalise(const GemmDesc& d0, hipStream_t st) {
     else {
         void* w = workspace((size_t)d.m * d.k * es, 0);
         pack_run(d.dt, d.a.ptr, d.m, d.k, d.a.rs, d.a.cs, w, st);
-        p.a = w; p.lda = d.m; p.a_k = false;
+        p.a = w; p.lda = d.m; p.a_k = false; p.a_packed = true;
     }
     if (d.b.rs == 1) { p.b = d.b.ptr; p.ldb = d.b.cs; p.b_k = true; }
     else if (d.b.cs == 1) { p.b = d.b.ptr; p.ldb = d.b.rs; p.b_k = false; }
     else {
         void* w = workspace((size_t)d.k * d.n * es, 1);
         pack_run(d.dt, d.b.ptr, d.k, d.n, d.b.rs, d.b.cs, w, st);
-        p.b = w; p.ldb = d.k; p.b_k = true;
-    }
-    // Large problems: k-contiguous operands to the rows-contiguous path.
-    if (p.a_k && d.a.cs == 1 && transpose_pays(d.dt, d.n, d.m, d.k)) {
-        const int64_t ldw = (d.m + 3) / 4 * 4;
-        void* w = workspace((size_t)ldw * d.k * es, 0);
-        transpose_run(d.dt, p.a, d.m, d.k, p.lda, w, ldw, st);
-        p.a = w; p.lda = ldw; p.a_k = false;
-    }
-    if (p.b_k && d.b.rs == 1 && transpose_pays(d.dt, d.m, d.n, d.k)) {
-        const int64_t ldw = (d.n + 3) / 4 * 4;
-        void* w = workspace((size_t)ldw * d.k * es, 1);
-        transpose_run(d.dt, p.b, d.n, d.k, p.ldb, w, ldw, st);
-        p.b = w; p.ldb = ldw; p.b_k = false;
+        p.b = w; p.ldb = d.k; p.b_k = true; p.b_packed = true;
     }
     return p;
+}
+
+// k-contiguous operands of a large product to the rows-contiguous path, one
+// transpose each into workspace (operands already packed are left alone).
+static void transpose_k_operands(Plan& p, Dt dt, hipStream_t st) {
+    const size_t es = dt_size(dt);
+    if (p.a_k && !p.a_packed && transpose_pays(dt, p.n, p.m, p.k)) {
+        const int64_t ldw = (p.m + 3) / 4 * 4;
+        void* w = workspace((size_t)ldw * p.k * es, 0);
+        transpose_run(dt, p.a, p.m, p.k, p.lda, w, ldw, st);
+        p.a = w; p.lda = ldw; p.a_k = false;
+    }
+    if (p.b_k && !p.b_packed && transpose_pays(dt, p.m, p.n, p.k)) {
+        const int64_t ldw = (p.n + 3) / 4 * 4;
+        void* w = workspace((size_t)ldw * p.k * es, 1);
+        transpose_run(dt, p.b, p.n, p.k, p.ldb, w, ldw, st);
+        p.b = w; p.ldb = ldw; p.b_k = false;
+    }
 }
 
 template <typename T>
@@ -329,6 +339,7 @@ void gemm_run(const GemmDesc& d, const GemmConfig* cfg, hipStream_t st) {
     g.xcd_remap = cfg->xcd_remap;
     int tile = cfg->tile >= 0 ? cfg->tile : gemm_default_tile(d.dt, pl.m, pl.n, pl.k);
     if (tile < 0 || tile >= tile_count(d.dt)) abort_with(BLIS_INVALID_UKR_ID, "gemm: bad tile");
+    if (d.dt == Dt::F64 && tile == 13) transpose_k_operands(pl, d.dt, st);
     if (d.dt == Dt::F64)
         launch_tile<double>(tile, make_problem<double>(pl, d.alpha, d.beta), pl.a_k, pl.b_k, g, st);
     else

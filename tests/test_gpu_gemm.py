@@ -220,13 +220,26 @@ def test_gemm_large_probe_residual(blis, dtype, shape):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_gemm_k_contiguous_operands_match_rows_contiguous(blis, dtype):
-    """Large GEMMs move a k-contiguous operand (op(A) = A^T, op(B) = B) to the
-    rows-contiguous layout before the tile kernel runs. The result must be
-    bit-identical to the same product given with the other layout, at ragged
-    sizes and padded leading dimensions (the 64 x 64 transpose tiles have
-    partial edges), and close to an FP64 product on sampled rows."""
+    """A k-contiguous operand (op(A) = A^T, op(B) = B) and a rows-contiguous
+    one must give bit-identical products, at ragged sizes and padded leading
+    dimensions, close to an FP64 product on sampled rows. For FP64 the
+    128 x 128 tile of sixteen 32 x 32 waves is pinned: on it, large products
+    move a k-contiguous operand to the rows-contiguous layout with one
+    transpose (64 x 64 tiles with partial edges here) before the tile kernel
+    runs."""
     import torch
     m, n, k = 6000, 4100, 3001
+    if np.dtype(dtype) == np.float64:
+        blis.bli_amd_gemm_set_config.argtypes = [C.c_int] * 4
+        blis.bli_amd_gemm_set_config(13, 8, 1, 1)
+    try:
+        _k_contiguous_case(blis, dtype, m, n, k)
+    finally:
+        blis.bli_amd_gemm_set_config(-1, 8, 1, 1)
+
+
+def _k_contiguous_case(blis, dtype, m, n, k):
+    import torch
     dev = torch.device("cuda")
     g = torch.Generator(device="cpu").manual_seed(77)
     A = (torch.rand(m, k, generator=g, dtype=torch.float64) * 2 - 1).to(dtype=_torch_dt(dtype))
