@@ -53,6 +53,45 @@ __global__ __launch_bounds__(TL::NT) void gemm_kernel(TileProblem<T> p, GridInfo
     gemm_tile<T, TL, A_K, B_K>(p, tm, tn, smem);
 }
 
+// One k partition of a split-k product per blockIdx.y: partition z covers
+// k in [z*kchunk, min(k, (z+1)*kchunk)) and stores alpha * A_z * B_z, without
+// reading anything there first, into its own m x n column-major slice of
+// `parts` (ld = m). splitk_reduce_kernel then forms C.
+template <typename T, class TL, bool A_K, bool B_K>
+__global__ __launch_bounds__(TL::NT) void gemm_splitk_kernel(TileProblem<T> p, GridInfo g,
+                                                             int64_t kchunk, T* parts) {
+    __shared__ T smem[2 * TL::STAGE];
+    const int64_t z = blockIdx.y, k0 = z * kchunk;
+    p.a += A_K ? k0 : k0 * p.lda;
+    p.b += B_K ? k0 : k0 * p.ldb;
+    p.k = min(kchunk, p.k - k0);
+    p.c = parts + z * p.m * p.n;
+    p.c_si = 1;
+    p.c_sj = p.m;
+    p.beta = T(0);
+    int tm, tn;
+    tile_coords(blockIdx.x, gridDim.x, g, tm, tn);
+    gemm_tile<T, TL, A_K, B_K>(p, tm, tn, smem);
+}
+
+// C := beta*C + sum_z parts_z, partitions added in ascending z (a fixed
+// order, so the result does not depend on scheduling). beta == 0 does not
+// read C.
+template <typename T>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const T* __restrict__ parts, int s,
+                                                            int64_t m, int64_t n, T* c,
+                                                            int64_t c_si, int64_t c_sj, T beta) {
+    const int64_t total = m * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        T acc = parts[t];
+        for (int z = 1; z < s; ++z) acc += parts[z * total + t];
+        const int64_t i = t % m, j = t / m;
+        T* dst = c + i * c_si + j * c_sj;
+        *dst = (beta == T(0)) ? acc : beta * *dst + acc;
+    }
+}
+
 // C := beta*C for alpha == 0 or k == 0 (bli_l3_return_early_if_trivial,
 // frame/3/bli_l3_util.c:41-65). beta == 0 stores zeros without reading C.
 template <typename T>
@@ -139,10 +178,36 @@ static void pack_run(Dt dt, const void* src, int64_t m, int64_t n, int64_t rs, i
 // Launch plumbing
 // ---------------------------------------------------------------------------
 
+// Split-k launch parameters: `parts` partitions of `kchunk` (the last one
+// shorter) into the workspace `ws`.
+struct SplitK {
+    int parts = 1;
+    int64_t kchunk = 0;
+    void* ws = nullptr;
+};
+
 template <typename T, class TL>
 static void launch_tl(const TileProblem<T>& p, bool a_k, bool b_k, const GridInfo& g,
-                      int nblocks, hipStream_t st) {
+                      int nblocks, const SplitK& sk, hipStream_t st) {
     dim3 grid(nblocks), block(TL::NT);
+    if (sk.parts > 1) {
+        const dim3 grid2(nblocks, sk.parts);
+        T* ws = (T*)sk.ws;
+        if (!a_k && !b_k)
+            hipLaunchKernelGGL((gemm_splitk_kernel<T, TL, false, false>), grid2, block, 0, st, p, g,
+                               sk.kchunk, ws);
+        else if (!a_k && b_k)
+            hipLaunchKernelGGL((gemm_splitk_kernel<T, TL, false, true>), grid2, block, 0, st, p, g,
+                               sk.kchunk, ws);
+        else if (a_k && !b_k)
+            hipLaunchKernelGGL((gemm_splitk_kernel<T, TL, true, false>), grid2, block, 0, st, p, g,
+                               sk.kchunk, ws);
+        else
+            hipLaunchKernelGGL((gemm_splitk_kernel<T, TL, true, true>), grid2, block, 0, st, p, g,
+                               sk.kchunk, ws);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
     if (!a_k && !b_k)
         hipLaunchKernelGGL((gemm_kernel<T, TL, false, false>), grid, block, 0, st, p, g);
     else if (!a_k && b_k)
@@ -156,7 +221,7 @@ static void launch_tl(const TileProblem<T>& p, bool a_k, bool b_k, const GridInf
 
 template <typename T>
 static void launch_tile(int tile, const TileProblem<T>& p, bool a_k, bool b_k,
-                        GridInfo g, hipStream_t st) {
+                        GridInfo g, const SplitK& sk, hipStream_t st) {
     const TileShape& ts = tile_shape(dt_of<T>(), tile);
     g.grid_m = (int)((p.m + ts.bm - 1) / ts.bm);
     g.grid_n = (int)((p.n + ts.bn - 1) / ts.bn);
@@ -165,14 +230,14 @@ static void launch_tile(int tile, const TileProblem<T>& p, bool a_k, bool b_k,
     if constexpr (std::is_same<T, double>::value) {
         switch (tile) {
 #define BLIS_AMD_CASE(idx, BM, BN, BK, WM, WN) \
-        case idx: launch_tl<double, TileCfg<double, BM, BN, BK, WM, WN>>(p, a_k, b_k, g, nblocks, st); return;
+        case idx: launch_tl<double, TileCfg<double, BM, BN, BK, WM, WN>>(p, a_k, b_k, g, nblocks, sk, st); return;
         BLIS_AMD_TILES_F64(BLIS_AMD_CASE)
 #undef BLIS_AMD_CASE
         }
     } else {
         switch (tile) {
 #define BLIS_AMD_CASE(idx, BM, BN, BK, WM, WN) \
-        case idx: launch_tl<float, TileCfg<float, BM, BN, BK, WM, WN>>(p, a_k, b_k, g, nblocks, st); return;
+        case idx: launch_tl<float, TileCfg<float, BM, BN, BK, WM, WN>>(p, a_k, b_k, g, nblocks, sk, st); return;
         BLIS_AMD_TILES_F32(BLIS_AMD_CASE)
 #undef BLIS_AMD_CASE
         }
@@ -311,6 +376,23 @@ int gemm_default_tile(Dt dt, int64_t m, int64_t n, int64_t k) {
     return 4;
 }
 
+// Number of k partitions for a product whose output is small against its
+// depth (m, n <= k/4, k >= 2048): such products have too few output tiles to
+// fill 256 CUs and a long serial k loop in each. Measured on MI355X with the
+// default tile (FP64; FP32 alike): 64x64x262144 7.64 ms -> 0.28 ms at 32
+// partitions, 128x128x65536 1.95 -> 0.09 ms, 256x256x32768 0.97 -> 0.11 ms,
+// 512x512x16384 0.49 -> 0.20 ms, 1024x1024x8192 0.42 -> 0.38 ms at 2;
+// 2048x2048x4096 gains nothing. The count aims at about 2048 workgroups of
+// 32x32 outputs and keeps every partition at least 1024 deep. It depends on
+// the shape only, so every tile gives the same partition.
+static int gemm_default_split(int64_t m, int64_t n, int64_t k) {
+    if (k < 2048 || 4 * std::max(m, n) > k) return 1;
+    const int64_t tiles = ((m + 31) / 32) * ((n + 31) / 32);
+    int64_t s = std::max<int64_t>(1, 2048 / tiles);
+    s = std::min<int64_t>(s, k / 1024);
+    return (int)std::min<int64_t>(s, 64);
+}
+
 // Process-wide override used when a caller passes no configuration
 // (set through bli_amd_gemm_set_config; tile -1 restores the heuristic).
 static GemmConfig g_override;
@@ -332,6 +414,8 @@ void gemm_run(const GemmDesc& d, const GemmConfig* cfg, hipStream_t st) {
         const bool vb = aligned16(pl.b) && pl.ldb % V == 0;
         const bool vc = aligned16(pl.c) && pl.c_si == 1 && pl.c_sj % V == 0;
         picked = config_of(selector_pick(d.dt, pl.m, pl.n, pl.k, pl.a_k, pl.b_k, va, vb, vc));
+        picked.split_k = g_override.split_k > 1 ? g_override.split_k
+                                                : gemm_default_split(pl.m, pl.n, pl.k);
         cfg = &picked;
     }
     GridInfo g{};
@@ -340,10 +424,40 @@ void gemm_run(const GemmDesc& d, const GemmConfig* cfg, hipStream_t st) {
     int tile = cfg->tile >= 0 ? cfg->tile : gemm_default_tile(d.dt, pl.m, pl.n, pl.k);
     if (tile < 0 || tile >= tile_count(d.dt)) abort_with(BLIS_INVALID_UKR_ID, "gemm: bad tile");
     if (d.dt == Dt::F64 && tile == 13) transpose_k_operands(pl, d.dt, st);
-    if (d.dt == Dt::F64)
-        launch_tile<double>(tile, make_problem<double>(pl, d.alpha, d.beta), pl.a_k, pl.b_k, g, st);
-    else
-        launch_tile<float>(tile, make_problem<float>(pl, d.alpha, d.beta), pl.a_k, pl.b_k, g, st);
+    // Split-k: partitions of a whole number of 64-deep k blocks (every
+    // tile's BK divides 64, so the partition, and with it the rounding, is
+    // the same for every tile), each partition a full product into
+    // workspace slot 17, then one reduction into C.
+    SplitK sk;
+    if (cfg->split_k > 1) {
+        const int64_t blocks = (pl.k + 63) / 64;
+        const int64_t want = std::min(cfg->split_k, 64);
+        sk.kchunk = (blocks + want - 1) / want * 64;
+        sk.parts = (int)((pl.k + sk.kchunk - 1) / sk.kchunk);
+    }
+    const size_t es = dt_size(d.dt);
+    if (sk.parts > 1) sk.ws = workspace((size_t)sk.parts * pl.m * pl.n * es, 17);
+    if (d.dt == Dt::F64) {
+        TileProblem<double> p = make_problem<double>(pl, d.alpha, d.beta);
+        if (sk.parts > 1) p.vec_c = pl.m % Mma<double>::V == 0;
+        launch_tile<double>(tile, p, pl.a_k, pl.b_k, g, sk, st);
+    } else {
+        TileProblem<float> p = make_problem<float>(pl, d.alpha, d.beta);
+        if (sk.parts > 1) p.vec_c = pl.m % Mma<float>::V == 0;
+        launch_tile<float>(tile, p, pl.a_k, pl.b_k, g, sk, st);
+    }
+    if (sk.parts > 1) {
+        const int gr = grid_for(pl.m * pl.n);
+        if (d.dt == Dt::F64)
+            hipLaunchKernelGGL(splitk_reduce_kernel<double>, dim3(gr), dim3(256), 0, st,
+                               (const double*)sk.ws, sk.parts, pl.m, pl.n, (double*)pl.c, pl.c_si,
+                               pl.c_sj, d.beta);
+        else
+            hipLaunchKernelGGL(splitk_reduce_kernel<float>, dim3(gr), dim3(256), 0, st,
+                               (const float*)sk.ws, sk.parts, pl.m, pl.n, (float*)pl.c, pl.c_si,
+                               pl.c_sj, (float)d.beta);
+        HIP_CHECK(hipGetLastError());
+    }
 }
 
 } // namespace blis_amd

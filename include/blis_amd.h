@@ -27,9 +27,12 @@ void  bli_amd_synchronize( void );
 /* ---- GEMM configuration ------------------------------------------------
    Force the tile configuration used by subsequent GEMMs whose caller gives
    none (tile = index into the gfx950 tile table, -1 = built-in heuristic),
-   the raster group height, the XCD remapping switch and the k split. The
-   arithmetic of a given tile index is fixed, so results depend only on the
-   tile index, never on how it was chosen. */
+   the raster group height, the XCD remapping switch and the k split
+   (split_k > 1: up to 64 partitions of whole 64-deep k blocks, each computed
+   into library workspace, then added in partition order and combined with
+   beta*C; 1 = one pass, or the built-in rule when tile is -1, which splits
+   only products with m, n <= k/4 and k >= 2048). Results depend on the k
+   partition only, never on the tile index or on how it was chosen. */
 void bli_amd_gemm_set_config( int tile, int group_m, int xcd_remap, int split_k );
 int  bli_amd_gemm_tile_count( int is_double );
 int  bli_amd_gemm_default_tile( int is_double, long long m, long long n, long long k );

@@ -187,6 +187,66 @@ def test_gemm_every_tile(blis, oracle, tile):
         blis.bli_amd_gemm_set_config(-1, 8, 1, 1)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("layout", ["col", "row", "padded", "general"])
+def test_gemm_split_k(blis, oracle, dtype, layout):
+    """k partitions into workspace plus one ordered reduction: the same
+    product as one pass, for ragged k, every transpose pair and every
+    storage of C."""
+    blis.bli_amd_gemm_set_config.argtypes = [C.c_int] * 4
+    try:
+        for tile, split, (m, n, k) in [(4, 3, (67, 45, 333)), (3, 8, (130, 65, 1000)),
+                                       (8, 2, (5, 129, 31)), (1, 64, (40, 30, 20))]:
+            blis.bli_amd_gemm_set_config(tile, 8, 1, split)
+            for ta, tb in [("n", "n"), ("t", "n"), ("n", "t"), ("t", "t")]:
+                check_parity(blis, oracle, dtype, ta, tb, m, n, k, -0.5, 0.75, layout)
+    finally:
+        blis.bli_amd_gemm_set_config(-1, 8, 1, 1)
+
+
+def test_gemm_split_k_beta_zero_over_nan(blis):
+    """The reduction stores C without reading it when beta == 0."""
+    blis.bli_amd_gemm_set_config.argtypes = [C.c_int] * 4
+    m, n, k = 70, 33, 500
+    say("dgemm split-k beta=0 over NaN")
+    a, b = Mat(m, k, seed=5).to_device(), Mat(k, n, seed=6).to_device()
+    c = Mat(m, n, fill=np.nan).to_device()
+    al, be = C.c_double(1.0), C.c_double(0.0)
+    try:
+        blis.bli_amd_gemm_set_config(3, 8, 1, 4)
+        blis.bli_dgemm(0, 0, m, n, k, C.byref(al), a.ptr, a.rs, a.cs, b.ptr, b.rs, b.cs,
+                       C.byref(be), c.ptr, c.rs, c.cs)
+    finally:
+        blis.bli_amd_gemm_set_config(-1, 8, 1, 1)
+    got = c.device_view()
+    assert np.isfinite(got).all()
+    np.testing.assert_allclose(got, a.view() @ b.view(), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_gemm_default_split_for_deep_products(blis, oracle, dtype):
+    """A small output over a long k (m, n <= k/4, k >= 2048) is split by the
+    built-in rule (here into 8 partitions); the result matches the oracle,
+    and every tile given the same split gives bitwise the same result."""
+    blis.bli_amd_gemm_set_config.argtypes = [C.c_int] * 4
+    m, n, k = 64, 48, 8192
+    check_parity(blis, oracle, dtype, "t", "n", m, n, k, 1.25, -0.5)
+    a, b, c = make_operands(dtype, "n", "t", m, n, k, 31)
+    for x in (a, b):
+        x.to_device()
+    outs = []
+    try:
+        for tile in (-1, 0, 1, 2, 3, 4):
+            blis.bli_amd_gemm_set_config(tile, 8, 1, 8 if tile >= 0 else 1)
+            cc = Mat(m, n, dtype=dtype, seed=33).to_device()
+            run_gemm(blis, dtype, "n", "t", m, n, k, 1.0, 0.5, a, b, cc)
+            outs.append(cc.device_view().copy())
+    finally:
+        blis.bli_amd_gemm_set_config(-1, 8, 1, 1)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
 def probe_resid(a, b, c_hat, c0, alpha, beta, ta, tb, seed=0):
     rng = np.random.default_rng(seed)
     t = rng.uniform(-1, 1, c_hat.shape[1])
