@@ -185,6 +185,15 @@ def extras(bench: Bench):
                 del fn
                 torch.cuda.empty_cache()
 
+    # Small outputs over a long k: the default path splits k (the built-in
+    # rule picks 64 partitions for 128x128x65536 and 2 for 1024x1024x8192).
+    for (m, n, k) in ((128, 128, 65536), (1024, 1024, 8192)):
+        fn = bench.gemm_fn("d", 0, 0, m, n, k)
+        ms = bench.kernel_ms(fn, 10)
+        rec(f"dgemm_nn_{m}x{n}x{k}", ms, 2.0 * m * n * k, "TFLOP/s", PEAK_TFLOPS["f64"], "mfma")
+        del fn
+        torch.cuda.empty_cache()
+
     # DTRSM left/lower/no-trans/non-unit, m = 4096, n = 1024.
     m, n = 4096, 1024
     rec("dtrsm_llnn_4096x1024", trsm_ms(bench), float(m) * m * n, "TFLOP/s", PEAK_TFLOPS["f64"],

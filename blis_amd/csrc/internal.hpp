@@ -56,7 +56,8 @@ BLIS_AMD_HIDDEN int         device_id();
 BLIS_AMD_HIDDEN void        sync_if_needed();
 // Grow-only device scratch, one buffer per slot: 0-1 GEMM operand packing
 // and transposes, 2-5 split-precision slices and statistics, 5-7 level-1/2
-// partials, 8-10 autotuner operands, 11-12 batch autotuner, 13-17 TRSM.
+// partials, 8-10 autotuner operands, 11-12 batch autotuner, 13-16 TRSM,
+// 17 GEMM split-k partitions.
 constexpr int WS_SLOTS = 20;
 BLIS_AMD_HIDDEN void*       workspace(size_t bytes, int slot);
 BLIS_AMD_HIDDEN bool        synchronous();   // calls wait for their results
