@@ -204,39 +204,71 @@ __global__ __launch_bounds__(256) void slice_kernel(const T* __restrict__ x, int
                                                     int64_t s_row, int64_t s_k, int row_contig,
                                                     const int* __restrict__ emax, int S,
                                                     signed char* __restrict__ out) {
-    __shared__ __attribute__((aligned(16))) signed char blk[SLICE_LDS];
+    // 64 * S * KC bytes, sized at launch so that fewer slices let more
+    // workgroups share a CU.
+    extern __shared__ __attribute__((aligned(16))) signed char blk[];
     const int tid = threadIdx.x;
-    const int r = row_contig ? (tid & 63) : (tid >> 2);
-    const int sub = row_contig ? (tid >> 6) : (tid & 3);
     const int64_t rblocks = rows_pad / 64, tiles = rblocks * nkc;
     const int chunks = 64 * S * KC / 16;                 // 16-byte chunks per block
     for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const int64_t rb = tile % rblocks, kc = tile / rblocks;
-        const int64_t i = rb * 64 + r;
         double v[16];
-        if (i < rows) {
-            const int em = emax[i];
-            const int e = (em == INT_MIN) ? 0 : em;
-            const int64_t pb = kc * KC + sub * 16;
+        if (row_contig) {
+            // Lanes along rows (unit stride): thread (r, sub) takes k
+            // positions sub*16 .. sub*16 + 15 of row r and writes its 16
+            // digits of each slice as one 16-byte chunk.
+            const int r = tid & 63, sub = tid >> 6;
+            const int64_t i = rb * 64 + r;
+            if (i < rows) {
+                const int em = emax[i];
+                const int e = (em == INT_MIN) ? 0 : em;
+                const int64_t pb = kc * KC + sub * 16;
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int64_t p = pb + q;
-                v[q] = (p < K) ? ldexp((double)x[i * s_row + p * s_k], -e) : 0.0;  // exact
+                for (int q = 0; q < 16; ++q) {
+                    const int64_t p = pb + q;
+                    v[q] = (p < K) ? ldexp((double)x[i * s_row + p * s_k], -e) : 0.0;  // exact
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) v[q] = 0.0;
+            }
+            for (int t = 0; t < S; ++t) {
+                union { signed char b[16]; i4_t w; } dg;
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const double y = v[q] * 128.0;    // exact
+                    const double tr = trunc(y);       // |tr| <= 127
+                    dg.b[q] = (signed char)(int)tr;
+                    v[q] = y - tr;                    // exact, |v| < 1
+                }
+                *reinterpret_cast<i4_t*>(blk + ((r * S + t) * KC + sub * 16)) = dg.w;
             }
         } else {
-#pragma unroll
-            for (int q = 0; q < 16; ++q) v[q] = 0.0;
-        }
-        for (int t = 0; t < S; ++t) {
-            union { signed char b[16]; i4_t w; } dg;
+            // Lanes along k (unit stride): lane l takes k position l of rows
+            // w*16 .. w*16 + 15 (w = wave), so each load instruction reads 64
+            // consecutive elements of one row. Digits go to LDS one byte each.
+            const int lane = tid & 63, w = tid >> 6;
+            const int64_t p = kc * KC + lane;
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
-                const double y = v[q] * 128.0;    // exact
-                const double tr = trunc(y);       // |tr| <= 127
-                dg.b[q] = (signed char)(int)tr;
-                v[q] = y - tr;                    // exact, |v| < 1
+                const int64_t i = rb * 64 + w * 16 + q;
+                double val = 0.0;
+                if (i < rows && p < K) {
+                    const int em = emax[i];
+                    const int e = (em == INT_MIN) ? 0 : em;
+                    val = ldexp((double)x[i * s_row + p * s_k], -e);              // exact
+                }
+                v[q] = val;
             }
-            *reinterpret_cast<i4_t*>(blk + ((r * S + t) * KC + sub * 16)) = dg.w;
+            for (int t = 0; t < S; ++t) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const double y = v[q] * 128.0;    // exact
+                    const double tr = trunc(y);       // |tr| <= 127
+                    blk[((w * 16 + q) * S + t) * KC + lane] = (signed char)(int)tr;
+                    v[q] = y - tr;                    // exact, |v| < 1
+                }
+            }
         }
         __syncthreads();
         i4_t* dst = reinterpret_cast<i4_t*>(out + (kc * rows_pad + rb * 64) * S * KC);
@@ -487,13 +519,16 @@ static void launch_slice(Dt dt, View v, int64_t rows, int64_t rows_pad, int64_t 
     const int64_t tiles = rows_pad / 64 * nkc;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(tiles, 8192));
     const int row_contig = (s_row == 1) ? 1 : 0;
+    const size_t lds = (size_t)64 * S * KC;
+    if (S < 1 || lds > (size_t)SLICE_LDS) abort_with(BLIS_INVALID_UKR_ID, "split: slice count");
     if (dt == Dt::F64)
-        hipLaunchKernelGGL(slice_kernel<double>, dim3(grid), dim3(256), 0, st,
+        hipLaunchKernelGGL(slice_kernel<double>, dim3(grid), dim3(256), lds, st,
                            (const double*)v.ptr, rows, rows_pad, K, nkc, s_row, s_k, row_contig,
                            emax, S, out);
     else
-        hipLaunchKernelGGL(slice_kernel<float>, dim3(grid), dim3(256), 0, st, (const float*)v.ptr,
-                           rows, rows_pad, K, nkc, s_row, s_k, row_contig, emax, S, out);
+        hipLaunchKernelGGL(slice_kernel<float>, dim3(grid), dim3(256), lds, st,
+                           (const float*)v.ptr, rows, rows_pad, K, nkc, s_row, s_k, row_contig,
+                           emax, S, out);
     HIP_CHECK(hipGetLastError());
 }
 
