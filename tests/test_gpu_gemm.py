@@ -195,8 +195,10 @@ def test_gemm_split_k(blis, oracle, dtype, layout):
     storage of C."""
     blis.bli_amd_gemm_set_config.argtypes = [C.c_int] * 4
     try:
+        # tile -1: the heuristic picks the tile, the forced split still holds
         for tile, split, (m, n, k) in [(4, 3, (67, 45, 333)), (3, 8, (130, 65, 1000)),
-                                       (8, 2, (5, 129, 31)), (1, 64, (40, 30, 20))]:
+                                       (8, 2, (5, 129, 31)), (1, 64, (40, 30, 20)),
+                                       (-1, 5, (90, 70, 640))]:
             blis.bli_amd_gemm_set_config(tile, 8, 1, split)
             for ta, tb in [("n", "n"), ("t", "n"), ("n", "t"), ("t", "t")]:
                 check_parity(blis, oracle, dtype, ta, tb, m, n, k, -0.5, 0.75, layout)
