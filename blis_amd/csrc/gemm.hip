@@ -74,6 +74,22 @@ __global__ __launch_bounds__(TL::NT) void gemm_splitk_kernel(TileProblem<T> p, G
     gemm_tile<T, TL, A_K, B_K>(p, tm, tn, smem);
 }
 
+// count products at fixed element strides, one per blockIdx.y:
+// C_z := beta*C_z + alpha*A_z*B_z with A_z = A + z*sa, B_z = B + z*sb,
+// C_z = C + z*sc.
+template <typename T, class TL, bool A_K, bool B_K>
+__global__ __launch_bounds__(TL::NT) void gemm_strided_kernel(TileProblem<T> p, GridInfo g,
+                                                              int64_t sa, int64_t sb, int64_t sc) {
+    __shared__ T smem[2 * TL::STAGE];
+    const int64_t z = blockIdx.y;
+    p.a += z * sa;
+    p.b += z * sb;
+    p.c += z * sc;
+    int tm, tn;
+    tile_coords(blockIdx.x, gridDim.x, g, tm, tn);
+    gemm_tile<T, TL, A_K, B_K>(p, tm, tn, smem);
+}
+
 // C := beta*C + sum_z parts_z, partitions added in ascending z (a fixed
 // order, so the result does not depend on scheduling). beta == 0 does not
 // read C.
@@ -457,6 +473,59 @@ void gemm_run(const GemmDesc& d, const GemmConfig* cfg, hipStream_t st) {
                                (const float*)sk.ws, sk.parts, pl.m, pl.n, (float*)pl.c, pl.c_si,
                                pl.c_sj, (float)d.beta);
         HIP_CHECK(hipGetLastError());
+    }
+}
+
+template <typename T, class TL>
+static void launch_strided(const TileProblem<T>& p, bool a_k, bool b_k, const GridInfo& g,
+                           int count, int64_t sa, int64_t sb, int64_t sc, hipStream_t st) {
+    const dim3 grid(g.grid_m * g.grid_n, count), block(TL::NT);
+    if (!a_k && !b_k)
+        hipLaunchKernelGGL((gemm_strided_kernel<T, TL, false, false>), grid, block, 0, st, p, g,
+                           sa, sb, sc);
+    else if (!a_k && b_k)
+        hipLaunchKernelGGL((gemm_strided_kernel<T, TL, false, true>), grid, block, 0, st, p, g,
+                           sa, sb, sc);
+    else if (a_k && !b_k)
+        hipLaunchKernelGGL((gemm_strided_kernel<T, TL, true, false>), grid, block, 0, st, p, g,
+                           sa, sb, sc);
+    else
+        hipLaunchKernelGGL((gemm_strided_kernel<T, TL, true, true>), grid, block, 0, st, p, g,
+                           sa, sb, sc);
+    HIP_CHECK(hipGetLastError());
+}
+
+void gemm_strided_run(const GemmDesc& d, int count, int64_t sa, int64_t sb, int64_t sc,
+                      hipStream_t st) {
+    if (d.m <= 0 || d.n <= 0 || count <= 0) return;
+    if (d.c.rs != 1 || d.k <= 0 || d.alpha == 0.0)
+        abort_with(BLIS_INVALID_UKR_ID, "gemm strided: C must be column-stored, k > 0, alpha != 0");
+    Plan pl = normalise(d, st);
+    if (pl.a_packed || pl.b_packed)
+        abort_with(BLIS_INVALID_UKR_ID, "gemm strided: operands need a unit stride");
+    // The 32x32 tile (tile 4 of both tables): these batches are of small
+    // products, several hundred tiles in all.
+    GridInfo g{};
+    g.group_m = 8;
+    g.xcd_remap = 1;
+    g.grid_m = (int)((pl.m + 31) / 32);
+    g.grid_n = (int)((pl.n + 31) / 32);
+    if (d.dt == Dt::F64) {
+        constexpr int V = Mma<double>::V;
+        TileProblem<double> p = make_problem<double>(pl, d.alpha, d.beta);
+        p.vec_a = p.vec_a && sa % V == 0;
+        p.vec_b = p.vec_b && sb % V == 0;
+        p.vec_c = p.vec_c && sc % V == 0;
+        launch_strided<double, TileCfg<double, 32, 32, 16, 16, 16>>(p, pl.a_k, pl.b_k, g, count,
+                                                                   sa, sb, sc, st);
+    } else {
+        constexpr int V = Mma<float>::V;
+        TileProblem<float> p = make_problem<float>(pl, d.alpha, d.beta);
+        p.vec_a = p.vec_a && sa % V == 0;
+        p.vec_b = p.vec_b && sb % V == 0;
+        p.vec_c = p.vec_c && sc % V == 0;
+        launch_strided<float, TileCfg<float, 32, 32, 32, 16, 16>>(p, pl.a_k, pl.b_k, g, count,
+                                                                 sa, sb, sc, st);
     }
 }
 

@@ -57,7 +57,7 @@ BLIS_AMD_HIDDEN void        sync_if_needed();
 // Grow-only device scratch, one buffer per slot: 0-1 GEMM operand packing
 // and transposes, 2-5 split-precision slices and statistics, 5-7 level-1/2
 // partials, 8-10 autotuner operands, 11-12 batch autotuner, 13-16 TRSM,
-// 17 GEMM split-k partitions.
+// 17 GEMM split-k partitions, 18-19 TRSM 512-row inverses.
 constexpr int WS_SLOTS = 20;
 BLIS_AMD_HIDDEN void*       workspace(size_t bytes, int slot);
 BLIS_AMD_HIDDEN bool        synchronous();   // calls wait for their results
@@ -90,6 +90,10 @@ BLIS_AMD_HIDDEN void stage_vec(Staged& s, const void* p, Dt dt, int64_t n,
 
 // ---- kernels' launchers ----
 BLIS_AMD_HIDDEN void gemm_run(const GemmDesc& d, const GemmConfig* cfg, hipStream_t st);
+// count products of one shape at fixed element strides between their A, B
+// and C (column-stored C; operands with a unit stride), in one launch.
+BLIS_AMD_HIDDEN void gemm_strided_run(const GemmDesc& d, int count, int64_t sa, int64_t sb,
+                                      int64_t sc, hipStream_t st);
 BLIS_AMD_HIDDEN void scalm_run(Dt dt, int64_t m, int64_t n, double beta, View c, hipStream_t st);
 BLIS_AMD_HIDDEN void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag,
                               int64_t m, int64_t n, double alpha, View a, View b,
