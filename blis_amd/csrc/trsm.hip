@@ -240,25 +240,26 @@ struct Solve {
     View rhs;
     View x;
     bool separate = false;
-    // Lower solves with every block inverted: inverses of the first npair
-    // aligned 512-row diagonal blocks (512 x 512 each, column-major, ld 512).
+    // Solves with every block inverted: inverses of the first npair aligned
+    // 512-row diagonal blocks (512 x 512 each, column-major, ld 512).
     const void* linv2 = nullptr;
     int64_t npair = 0;
 };
 
 // Diagonal quarters of the 512-row inverses from the 256-row ones (block
-// 2p at the top left, 2p+1 at the bottom right of pair p) and zeros above
-// them; the bottom-left quarter -inv(L22) L21 inv(L11) is written by GEMMs.
+// 2p at the top left, 2p+1 at the bottom right of pair p) and zeros in the
+// quarter outside the triangle; the other off-diagonal quarter is written by
+// GEMMs.
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_assemble_kernel(const T* __restrict__ linv,
                                                             T* __restrict__ linv2,
-                                                            int64_t npair) {
+                                                            int64_t npair, int upper) {
     constexpr int64_t PB = 2 * NB;
     const int64_t total = npair * PB * PB;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
          t += (int64_t)gridDim.x * blockDim.x) {
         const int64_t p = t / (PB * PB), e = t % (PB * PB), i = e % PB, j = e / PB;
-        if (i >= NB && j < NB) continue;
+        if (upper ? (i < NB && j >= NB) : (i >= NB && j < NB)) continue;
         T v = T(0);
         if (i < NB && j < NB) v = linv[(2 * p) * NB * NB + i + j * NB];
         else if (i >= NB && j >= NB) v = linv[(2 * p + 1) * NB * NB + (i - NB) + (j - NB) * NB];
@@ -406,13 +407,15 @@ void solve(const Solve& s, int64_t r0, int64_t m, double alpha) {
     }
 }
 
-// Inverses of the aligned 512-row diagonal blocks of a lower solve from the
-// 256-row inverses at linv (ld NB): inv([[L11, 0], [L21, L22]]) =
-// [[inv(L11), 0], [-inv(L22) L21 inv(L11), inv(L22)]]. Two strided launches
-// of 256^3 products cover every pair. The blocks passed the dominance
-// check, so inv(L11) has condition number at most 48 and the product form
-// of the 512-row leaf loses at most that factor against the leaf, update,
-// leaf sequence it replaces.
+// Inverses of the aligned 512-row diagonal blocks from the 256-row inverses
+// at linv (ld NB): inv([[L11, 0], [L21, L22]]) =
+// [[inv(L11), 0], [-inv(L22) L21 inv(L11), inv(L22)]] for a lower triangle,
+// inv([[U11, U12], [0, U22]]) = [[inv(U11), -inv(U11) U12 inv(U22)],
+// [0, inv(U22)]] for an upper one. Two strided launches of 256^3 products
+// cover every pair. The blocks passed the dominance check, so the inverse
+// applied first has condition number at most 48 and the product form of
+// the 512-row leaf loses at most that factor against the leaf, update, leaf
+// sequence it replaces.
 void compose_pairs(Solve& s, const void* linv, int64_t rows) {
     const size_t es = dt_size(s.dt);
     const int64_t npair = rows / (2 * NB);
@@ -421,27 +424,31 @@ void compose_pairs(Solve& s, const void* linv, int64_t rows) {
     void* t = workspace((size_t)npair * NB * NB * es, 19);
     const int64_t total = npair * 4 * NB * NB;
     const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 8192);
+    const int upper = s.lower ? 0 : 1;
     if (s.dt == Dt::F64)
         hipLaunchKernelGGL(trsm_assemble_kernel<double>, dim3(g), dim3(256), 0, s.st,
-                           (const double*)linv, (double*)linv2, npair);
+                           (const double*)linv, (double*)linv2, npair, upper);
     else
         hipLaunchKernelGGL(trsm_assemble_kernel<float>, dim3(g), dim3(256), 0, s.st,
-                           (const float*)linv, (float*)linv2, npair);
+                           (const float*)linv, (float*)linv2, npair, upper);
     HIP_CHECK(hipGetLastError());
-    // T_p = L21_p * inv(L11_p)
+    const size_t blk = (size_t)NB * NB * es;
+    // Lower: T_p = L21_p * inv(L11_p). Upper: T_p = U12_p * inv(U22_p).
     GemmDesc d;
     d.dt = s.dt;
     d.m = NB; d.n = NB; d.k = NB;
-    d.a = { offset(s.a, NB, 0, es), s.a.rs, s.a.cs };
-    d.b = { const_cast<void*>(linv), 1, NB };
+    d.a = upper ? View{ offset(s.a, 0, NB, es), s.a.rs, s.a.cs }
+                : View{ offset(s.a, NB, 0, es), s.a.rs, s.a.cs };
+    d.b = { (char*)const_cast<void*>(linv) + (upper ? blk : 0), 1, NB };
     d.c = { t, 1, NB };
     d.alpha = 1.0;
     d.beta = 0.0;
     gemm_strided_run(d, (int)npair, 2 * NB * (s.a.rs + s.a.cs), 2 * NB * NB, NB * NB, s.st);
-    // W_p = -inv(L22_p) * T_p into the bottom-left quarter of pair p
-    d.a = { (char*)linv + (size_t)NB * NB * es, 1, NB };
+    // Lower: -inv(L22_p) * T_p into the bottom-left quarter of pair p.
+    // Upper: -inv(U11_p) * T_p into the top-right quarter.
+    d.a = { (char*)const_cast<void*>(linv) + (upper ? 0 : blk), 1, NB };
     d.b = { t, 1, NB };
-    d.c = { (char*)linv2 + (size_t)NB * es, 1, 2 * NB };
+    d.c = { (char*)linv2 + (upper ? (size_t)NB * 2 * NB * es : (size_t)NB * es), 1, 2 * NB };
     d.alpha = -1.0;
     gemm_strided_run(d, (int)npair, 2 * NB * NB, NB * NB, 4 * NB * NB, s.st);
     s.linv2 = linv2;
@@ -568,7 +575,7 @@ void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag, int64
             // once at the end (no per-leaf staging copies).
             s.x = { workspace((size_t)rows * s.n * es, 16), 1, rows };
             s.separate = true;
-            if (s.lower && rows >= 2 * NB) compose_pairs(s, linv, rows);
+            if (rows >= 2 * NB) compose_pairs(s, linv, rows);
             solve(s, 0, rows, alpha);
             copy(s.x, s.b);
             return;

@@ -98,13 +98,17 @@ def test_dtrsm_f77_alpha_zero(blis):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_trsm_config2_residual(blis, dtype):
-    """Config 2 shape: left/lower/no-trans/non-unit, m=4096, n=1024, A with
-    diagonal 4096 + U[0,1], off-diagonal U[-1,1], alpha = 1."""
+@pytest.mark.parametrize("uplo", [LOWER, UPPER])
+def test_trsm_config2_residual(blis, dtype, uplo):
+    """Config 2 shape: left/no-trans/non-unit, m=4096, n=1024, A with
+    diagonal 4096 + U[0,1], off-diagonal U[-1,1], alpha = 1; lower as in the
+    benchmark, and upper (every aligned 512-row block solved by one product
+    with its composed inverse either way)."""
     m, n = 4096, 1024
-    say(f"trsm config2 {np.dtype(dtype).name}")
+    say(f"trsm config2 {np.dtype(dtype).name} uplo={uplo:#x}")
     rng = np.random.default_rng(77)
-    full = np.tril(rng.uniform(-1, 1, (m, m)))
+    full = rng.uniform(-1, 1, (m, m))
+    full = np.tril(full) if uplo == LOWER else np.triu(full)
     full[np.diag_indices(m)] = 4096 + rng.uniform(0, 1, m)
     a = Mat(m, m, dtype=dtype)
     a.set(full.astype(dtype))
@@ -112,7 +116,7 @@ def test_trsm_config2_residual(blis, dtype):
     b0 = b.view().astype(np.float64)
     a.to_device()
     b.to_device()
-    run_trsm(blis, dtype, 0, LOWER, "n", 0, m, n, 1.0, a, b)
+    run_trsm(blis, dtype, 0, uplo, "n", 0, m, n, 1.0, a, b)
     x = b.device_view().astype(np.float64)
     af = a.view().astype(np.float64)
     resid = np.linalg.norm(af @ x - b0) / (np.linalg.norm(af) * np.linalg.norm(x) * n * unit_roundoff(dtype))
