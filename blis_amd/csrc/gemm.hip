@@ -269,7 +269,26 @@ struct Plan {
     const void* a; int64_t lda; bool a_k;
     const void* b; int64_t ldb; bool b_k;
     void* c; int64_t c_si, c_sj;
-    bool a_packed = false, b_packed = false;   // operand copied into workspace
+    bool a_packed = false, b_packed = false;   // operand copied into scratch
+    // Source of an operand that still has to be packed (general strides).
+    View a_src, b_src;
+};
+
+// Device memory a GEMM takes beyond its operands: the library's grow-only
+// workspace arena, or, when `base` is set, a fixed region handed in by the
+// caller, carved in 256-byte aligned pieces (nothing is allocated then).
+struct Scratch {
+    char*  base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    void* take(size_t bytes, int slot) {
+        if (base == nullptr) return workspace(bytes, slot);
+        const size_t start = (used + 255) & ~(size_t)255;
+        if (start + bytes > cap)
+            abort_with(BLIS_INSUFFICIENT_STACK_BUF_SIZE, "gemm: fixed scratch region too small");
+        used = start + bytes;
+        return base + start;
+    }
 };
 
 static void transpose_run(Dt dt, const void* src, int64_t rows, int64_t cols, int64_t lds,
@@ -301,7 +320,14 @@ static bool transpose_pays(Dt dt, int64_t reuse, int64_t rows, int64_t k) {
     return 0.04 * gemm_s > tr_s;
 }
 
-static Plan normalise(const GemmDesc& d0, hipStream_t st) {
+// Storage normalisation without side effects: row-stored C is recast as
+// C^T = B^T A^T, each operand is classified as rows- or k-contiguous, and an
+// operand with neither unit stride is marked for packing into a contiguous
+// column-major copy (A: m x k, ld m; B: k x n, ld k). pack_operands() then
+// makes those copies. Splitting the two lets the configuration be chosen
+// before any scratch memory is written (the exhaustive autotuner runs GEMMs
+// of its own that use the same scratch).
+static Plan layout(const GemmDesc& d0) {
     GemmDesc d = d0;
     // Row-stored C: compute C^T = B^T A^T instead.
     if (d.c.rs != 1 && d.c.cs == 1) {
@@ -315,37 +341,48 @@ static Plan normalise(const GemmDesc& d0, hipStream_t st) {
     Plan p;
     p.m = d.m; p.n = d.n; p.k = d.k;
     p.c = d.c.ptr; p.c_si = d.c.rs; p.c_sj = d.c.cs;
-    const size_t es = dt_size(d.dt);
     if (d.a.rs == 1) { p.a = d.a.ptr; p.lda = d.a.cs; p.a_k = false; }
     else if (d.a.cs == 1) { p.a = d.a.ptr; p.lda = d.a.rs; p.a_k = true; }
-    else {
-        void* w = workspace((size_t)d.m * d.k * es, 0);
-        pack_run(d.dt, d.a.ptr, d.m, d.k, d.a.rs, d.a.cs, w, st);
-        p.a = w; p.lda = d.m; p.a_k = false; p.a_packed = true;
-    }
+    else { p.a = nullptr; p.a_src = d.a; p.lda = d.m; p.a_k = false; p.a_packed = true; }
     if (d.b.rs == 1) { p.b = d.b.ptr; p.ldb = d.b.cs; p.b_k = true; }
     else if (d.b.cs == 1) { p.b = d.b.ptr; p.ldb = d.b.rs; p.b_k = false; }
-    else {
-        void* w = workspace((size_t)d.k * d.n * es, 1);
-        pack_run(d.dt, d.b.ptr, d.k, d.n, d.b.rs, d.b.cs, w, st);
-        p.b = w; p.ldb = d.k; p.b_k = true; p.b_packed = true;
-    }
+    else { p.b = nullptr; p.b_src = d.b; p.ldb = d.k; p.b_k = true; p.b_packed = true; }
     return p;
+}
+
+static void pack_operands(Plan& p, Dt dt, Scratch& sc, hipStream_t st) {
+    const size_t es = dt_size(dt);
+    if (p.a_packed && p.a == nullptr) {
+        void* w = sc.take((size_t)p.m * p.k * es, 0);
+        pack_run(dt, p.a_src.ptr, p.m, p.k, p.a_src.rs, p.a_src.cs, w, st);
+        p.a = w;
+    }
+    if (p.b_packed && p.b == nullptr) {
+        void* w = sc.take((size_t)p.k * p.n * es, 1);
+        pack_run(dt, p.b_src.ptr, p.k, p.n, p.b_src.rs, p.b_src.cs, w, st);
+        p.b = w;
+    }
+}
+
+// 16-byte vector access to an operand: packed copies start 256-byte aligned.
+static bool vec_ok(const void* ptr, bool packed, int64_t ld, Dt dt) {
+    const int64_t V = dt == Dt::F64 ? 2 : 4;
+    return (packed || aligned16(ptr)) && ld % V == 0;
 }
 
 // k-contiguous operands of a large product to the rows-contiguous path, one
 // transpose each into workspace (operands already packed are left alone).
-static void transpose_k_operands(Plan& p, Dt dt, hipStream_t st) {
+static void transpose_k_operands(Plan& p, Dt dt, Scratch& sc, hipStream_t st) {
     const size_t es = dt_size(dt);
     if (p.a_k && !p.a_packed && transpose_pays(dt, p.n, p.m, p.k)) {
         const int64_t ldw = (p.m + 3) / 4 * 4;
-        void* w = workspace((size_t)ldw * p.k * es, 0);
+        void* w = sc.take((size_t)ldw * p.k * es, 0);
         transpose_run(dt, p.a, p.m, p.k, p.lda, w, ldw, st);
         p.a = w; p.lda = ldw; p.a_k = false;
     }
     if (p.b_k && !p.b_packed && transpose_pays(dt, p.m, p.n, p.k)) {
         const int64_t ldw = (p.n + 3) / 4 * 4;
-        void* w = workspace((size_t)ldw * p.k * es, 1);
+        void* w = sc.take((size_t)ldw * p.k * es, 1);
         transpose_run(dt, p.b, p.n, p.k, p.ldb, w, ldw, st);
         p.b = w; p.ldb = ldw; p.b_k = false;
     }
@@ -413,6 +450,58 @@ static int gemm_default_split(int64_t m, int64_t n, int64_t k) {
 // (set through bli_amd_gemm_set_config; tile -1 restores the heuristic).
 static GemmConfig g_override;
 
+// Split-k launch parameters for `split` requested partitions of whole
+// 64-deep k blocks (every tile's BK divides 64, so the partition, and with
+// it the rounding, is the same for every tile).
+static SplitK split_params(int64_t k, int split) {
+    SplitK sk;
+    if (split > 1) {
+        const int64_t blocks = (k + 63) / 64;
+        const int64_t want = std::min(split, 64);
+        sk.kchunk = (blocks + want - 1) / want * 64;
+        sk.parts = (int)((k + sk.kchunk - 1) / sk.kchunk);
+    }
+    return sk;
+}
+
+// Launch a normalised, packed plan with configuration cfg: the tile kernel,
+// and for split-k the partition products into scratch and one reduction.
+static void launch_plan(Dt dt, Plan& pl, const GemmConfig& cfg, double alpha, double beta,
+                        Scratch& sc, hipStream_t st) {
+    GridInfo g{};
+    g.group_m = cfg.group_m;
+    g.xcd_remap = cfg.xcd_remap;
+    const int tile = cfg.tile >= 0 ? cfg.tile : gemm_default_tile(dt, pl.m, pl.n, pl.k);
+    if (tile < 0 || tile >= tile_count(dt)) abort_with(BLIS_INVALID_UKR_ID, "gemm: bad tile");
+    if (dt == Dt::F64 && tile == 13) transpose_k_operands(pl, dt, sc, st);
+    // Split-k: each partition a full product into its own m x n slice of
+    // scratch, then one reduction into C.
+    SplitK sk = split_params(pl.k, cfg.split_k);
+    const size_t es = dt_size(dt);
+    if (sk.parts > 1) sk.ws = sc.take((size_t)sk.parts * pl.m * pl.n * es, 17);
+    if (dt == Dt::F64) {
+        TileProblem<double> p = make_problem<double>(pl, alpha, beta);
+        if (sk.parts > 1) p.vec_c = pl.m % Mma<double>::V == 0;
+        launch_tile<double>(tile, p, pl.a_k, pl.b_k, g, sk, st);
+    } else {
+        TileProblem<float> p = make_problem<float>(pl, alpha, beta);
+        if (sk.parts > 1) p.vec_c = pl.m % Mma<float>::V == 0;
+        launch_tile<float>(tile, p, pl.a_k, pl.b_k, g, sk, st);
+    }
+    if (sk.parts > 1) {
+        const int gr = grid_for(pl.m * pl.n);
+        if (dt == Dt::F64)
+            hipLaunchKernelGGL(splitk_reduce_kernel<double>, dim3(gr), dim3(256), 0, st,
+                               (const double*)sk.ws, sk.parts, pl.m, pl.n, (double*)pl.c, pl.c_si,
+                               pl.c_sj, beta);
+        else
+            hipLaunchKernelGGL(splitk_reduce_kernel<float>, dim3(gr), dim3(256), 0, st,
+                               (const float*)sk.ws, sk.parts, pl.m, pl.n, (float*)pl.c, pl.c_si,
+                               pl.c_sj, (float)beta);
+        HIP_CHECK(hipGetLastError());
+    }
+}
+
 void gemm_run(const GemmDesc& d, const GemmConfig* cfg, hipStream_t st) {
     if (d.m <= 0 || d.n <= 0) return;
     if (cfg == nullptr) cfg = &g_override;
@@ -420,60 +509,23 @@ void gemm_run(const GemmDesc& d, const GemmConfig* cfg, hipStream_t st) {
         scalm_run(d.dt, d.m, d.n, d.beta, d.c, st);
         return;
     }
-    Plan pl = normalise(d, st);
+    Plan pl = layout(d);
     GemmConfig picked;
     if (cfg == &g_override && g_override.tile < 0) {
         // No configuration given: the selector (heuristic, learned or
-        // exhaustive) picks one from the normalised shape.
-        const int V = d.dt == Dt::F64 ? 2 : 4;
-        const bool va = aligned16(pl.a) && pl.lda % V == 0;
-        const bool vb = aligned16(pl.b) && pl.ldb % V == 0;
-        const bool vc = aligned16(pl.c) && pl.c_si == 1 && pl.c_sj % V == 0;
+        // exhaustive) picks one from the normalised shape, before anything
+        // is packed into scratch.
+        const bool va = vec_ok(pl.a, pl.a_packed, pl.lda, d.dt);
+        const bool vb = vec_ok(pl.b, pl.b_packed, pl.ldb, d.dt);
+        const bool vc = aligned16(pl.c) && pl.c_si == 1 && pl.c_sj % (d.dt == Dt::F64 ? 2 : 4) == 0;
         picked = config_of(selector_pick(d.dt, pl.m, pl.n, pl.k, pl.a_k, pl.b_k, va, vb, vc));
         picked.split_k = g_override.split_k > 1 ? g_override.split_k
                                                 : gemm_default_split(pl.m, pl.n, pl.k);
         cfg = &picked;
     }
-    GridInfo g{};
-    g.group_m = cfg->group_m;
-    g.xcd_remap = cfg->xcd_remap;
-    int tile = cfg->tile >= 0 ? cfg->tile : gemm_default_tile(d.dt, pl.m, pl.n, pl.k);
-    if (tile < 0 || tile >= tile_count(d.dt)) abort_with(BLIS_INVALID_UKR_ID, "gemm: bad tile");
-    if (d.dt == Dt::F64 && tile == 13) transpose_k_operands(pl, d.dt, st);
-    // Split-k: partitions of a whole number of 64-deep k blocks (every
-    // tile's BK divides 64, so the partition, and with it the rounding, is
-    // the same for every tile), each partition a full product into
-    // workspace slot 17, then one reduction into C.
-    SplitK sk;
-    if (cfg->split_k > 1) {
-        const int64_t blocks = (pl.k + 63) / 64;
-        const int64_t want = std::min(cfg->split_k, 64);
-        sk.kchunk = (blocks + want - 1) / want * 64;
-        sk.parts = (int)((pl.k + sk.kchunk - 1) / sk.kchunk);
-    }
-    const size_t es = dt_size(d.dt);
-    if (sk.parts > 1) sk.ws = workspace((size_t)sk.parts * pl.m * pl.n * es, 17);
-    if (d.dt == Dt::F64) {
-        TileProblem<double> p = make_problem<double>(pl, d.alpha, d.beta);
-        if (sk.parts > 1) p.vec_c = pl.m % Mma<double>::V == 0;
-        launch_tile<double>(tile, p, pl.a_k, pl.b_k, g, sk, st);
-    } else {
-        TileProblem<float> p = make_problem<float>(pl, d.alpha, d.beta);
-        if (sk.parts > 1) p.vec_c = pl.m % Mma<float>::V == 0;
-        launch_tile<float>(tile, p, pl.a_k, pl.b_k, g, sk, st);
-    }
-    if (sk.parts > 1) {
-        const int gr = grid_for(pl.m * pl.n);
-        if (d.dt == Dt::F64)
-            hipLaunchKernelGGL(splitk_reduce_kernel<double>, dim3(gr), dim3(256), 0, st,
-                               (const double*)sk.ws, sk.parts, pl.m, pl.n, (double*)pl.c, pl.c_si,
-                               pl.c_sj, d.beta);
-        else
-            hipLaunchKernelGGL(splitk_reduce_kernel<float>, dim3(gr), dim3(256), 0, st,
-                               (const float*)sk.ws, sk.parts, pl.m, pl.n, (float*)pl.c, pl.c_si,
-                               pl.c_sj, (float)d.beta);
-        HIP_CHECK(hipGetLastError());
-    }
+    Scratch sc;
+    pack_operands(pl, d.dt, sc, st);
+    launch_plan(d.dt, pl, *cfg, d.alpha, d.beta, sc, st);
 }
 
 template <typename T, class TL>
@@ -500,7 +552,7 @@ void gemm_strided_run(const GemmDesc& d, int count, int64_t sa, int64_t sb, int6
     if (d.m <= 0 || d.n <= 0 || count <= 0) return;
     if (d.c.rs != 1 || d.k <= 0 || d.alpha == 0.0)
         abort_with(BLIS_INVALID_UKR_ID, "gemm strided: C must be column-stored, k > 0, alpha != 0");
-    Plan pl = normalise(d, st);
+    Plan pl = layout(d);
     if (pl.a_packed || pl.b_packed)
         abort_with(BLIS_INVALID_UKR_ID, "gemm strided: operands need a unit stride");
     // The 32x32 tile (tile 4 of both tables): these batches are of small
@@ -533,6 +585,8 @@ void gemm_strided_run(const GemmDesc& d, int count, int64_t sa, int64_t sb, int6
 
 extern "C" __attribute__((visibility("default")))
 void bli_amd_gemm_set_config(int tile, int group_m, int xcd_remap, int split_k) {
+    blis_amd::init_once();
+    std::lock_guard<std::mutex> lock(blis_amd::api_mutex());
     blis_amd::g_override.tile = tile;
     blis_amd::g_override.group_m = group_m > 0 ? group_m : 8;
     blis_amd::g_override.xcd_remap = xcd_remap;

@@ -194,6 +194,8 @@ __attribute__((visibility("default")))
 void bli_amd_set_stream(void* hip_stream, int sync) {
     init_once();
     State& s = S();
+    // Calls in flight hold the API mutex for as long as they use the stream.
+    std::lock_guard<std::mutex> lock(s.api);
     static hipStream_t own = nullptr;
     if (!s.user_stream) own = s.stream;
     if (hip_stream) { s.stream = (hipStream_t)hip_stream; s.user_stream = true; }

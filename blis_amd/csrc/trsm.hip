@@ -121,8 +121,10 @@ __global__ __launch_bounds__(64 * WPB) void trsm_leaf_kernel(const T* __restrict
 #pragma unroll
             for (int c = 0; c < CPW; ++c) {
                 const T xi = lane_bcast(x[c], i) * inv_i;
+                // Rows solved earlier are left as they are (an Inf in xi
+                // must not turn them into NaN through a zero of L).
                 if (lane == i) x[c] = xi;
-                else x[c] -= Lr[i] * xi;
+                else if (LOWER ? lane > i : lane < i) x[c] -= Lr[i] * xi;
             }
         }
 #pragma unroll
@@ -155,11 +157,14 @@ __global__ __launch_bounds__(NB) void trsm_diag_check_kernel(const T* __restrict
                                                             int64_t acs, int64_t m,
                                                             int unit_diag, int* __restrict__ ok) {
     __shared__ int bad;
-    __shared__ float dmax, dmin;
+    // Largest and smallest |l_ii| as IEEE bit patterns: for non-negative
+    // doubles the unsigned order of the patterns is the numeric order, so
+    // the extremes are exact whatever the diagonal's range.
+    __shared__ unsigned long long dmax, dmin;
     const int t = threadIdx.x;
     const int64_t q0 = (int64_t)blockIdx.x * NB;
     const int64_t nb = (m - q0 < NB) ? m - q0 : NB;
-    if (t == 0) { bad = 0; dmax = 0.f; dmin = 3.0e38f; }
+    if (t == 0) { bad = 0; dmax = 0ull; dmin = ~0ull; }
     __syncthreads();
     if (t < nb) {
         const int64_t r = q0 + t;
@@ -176,11 +181,16 @@ __global__ __launch_bounds__(NB) void trsm_diag_check_kernel(const T* __restrict
                            ((part[4] + part[5]) + (part[6] + part[7]));
         const double d = unit_diag ? 1.0 : fabs((double)a[r * ars + r * acs]);
         if (!(d > 2.0 * off) || !(d < 1.0e300)) atomicOr(&bad, 1);
-        atomicMax(&dmax, (float)d);
-        atomicMin(&dmin, (float)d);
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(d);
+        atomicMax(&dmax, bits);
+        atomicMin(&dmin, bits);
     }
     __syncthreads();
-    if (t == 0) ok[blockIdx.x] = (!bad && dmax <= 16.0f * dmin) ? 1 : 0;
+    if (t == 0) {
+        const double hi = __longlong_as_double((long long)dmax);
+        const double lo = __longlong_as_double((long long)dmin);
+        ok[blockIdx.x] = (!bad && hi <= 16.0 * lo) ? 1 : 0;
+    }
 }
 
 // nblk identity blocks of NB x NB, column-major with leading dimension NB.
@@ -575,7 +585,11 @@ void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag, int64
             // once at the end (no per-leaf staging copies).
             s.x = { workspace((size_t)rows * s.n * es, 16), 1, rows };
             s.separate = true;
-            if (rows >= 2 * NB) compose_pairs(s, linv, rows);
+            // The composition multiplies off-diagonal blocks of the triangle
+            // in place, which needs a unit stride; a general-stride triangle
+            // keeps 256-row leaves with updates between them (the update
+            // GEMMs pack it).
+            if (rows >= 2 * NB && (s.a.rs == 1 || s.a.cs == 1)) compose_pairs(s, linv, rows);
             solve(s, 0, rows, alpha);
             copy(s.x, s.b);
             return;

@@ -236,3 +236,31 @@ def test_learned_mode_with_shipped_weights(blis, L):
         assert np.array_equal(c1.device_view(), ref)
     finally:
         L.bli_amd_gemm_set_selector(0, None)
+
+
+def test_exhaustive_mode_general_stride_b(blis, L):
+    """Exhaustive selection for a product whose B has general strides, at a
+    size where the autotuner's large-tile configurations transpose their own
+    operands through the library's scratch: the configuration is chosen
+    before the caller's B is packed, so the packed copy is not overwritten
+    and the result is bit-identical to the heuristic's."""
+    say("selector mode 2, general-stride B, 4096x3000x3000")
+    L.bli_amd_gemm_set_selector.argtypes = [C.c_int, C.c_void_p]
+    m, n, k = 4096, 3000, 3000
+    a = Mat(m, k, seed=25).to_device()
+    b = Mat(k, n, rs=2, cs=2 * k, seed=26).to_device()
+    c0 = Mat(m, n, seed=27).to_device()
+    al, be = C.c_double(1.0), C.c_double(0.5)
+
+    def run(c):
+        blis.bli_dgemm(0, 0, m, n, k, C.byref(al), a.ptr, a.rs, a.cs, b.ptr, b.rs, b.cs,
+                       C.byref(be), c.ptr, c.rs, c.cs)
+    run(c0)
+    ref = c0.device_view().copy()
+    try:
+        L.bli_amd_gemm_set_selector(2, None)
+        c1 = Mat(m, n, seed=27).to_device()
+        run(c1)
+        assert np.array_equal(c1.device_view(), ref)
+    finally:
+        L.bli_amd_gemm_set_selector(0, None)

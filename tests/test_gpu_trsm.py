@@ -202,3 +202,52 @@ def test_trsm_many_leaves_async(blis, oracle):
         oracle.trsm(0, LOWER, "n", 0, 2.0, a.view().copy(order="F"), ref)
         err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
         assert err <= 8 * max(m, n) * unit_roundoff(np.float64), (kind, err)
+
+
+@pytest.mark.parametrize("uplo", [LOWER, UPPER])
+@pytest.mark.parametrize("ta", ["n", "t"])
+def test_trsm_general_stride_dominant(blis, oracle, uplo, ta):
+    """A dominant triangle with neither a unit row nor a unit column stride,
+    large enough for 512-row blocks (700 rows): every diagonal block passes
+    the check, and the composed 512-row inverse, which multiplies
+    off-diagonal blocks in place, must not be used for such a triangle."""
+    m, n = 700, 530
+    say(f"dtrsm general-stride A uplo={uplo:#x} {ta} m={m} n={n}")
+    _, full = tri_matrix(m, uplo, False, np.float64, seed=m + 5)
+    a = Mat(m, m, rs=2, cs=2 * m, seed=3)
+    a.set(full)
+    b = Mat(m, n, seed=m + 3 * n)
+    b0 = b.view().copy()
+    a.to_device()
+    b.to_device()
+    run_trsm(blis, np.float64, 0, uplo, ta, 0, m, n, 2.0, a, b)
+    got = b.device_view()
+    ref = b0.copy(order="F")
+    oracle.trsm(0, uplo, ta, 0, 2.0, a.view().copy(order="F"), ref)
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert err <= 8 * max(m, n) * unit_roundoff(np.float64), err
+
+
+@pytest.mark.parametrize("uplo", [LOWER, UPPER])
+def test_trsm_inf_in_b_substitution(blis, oracle, uplo):
+    """An Inf in B on the substitution path: rows solved before it stay
+    finite and the Inf/NaN pattern of the rest follows the oracle (the
+    reference's trsm micro-kernel never updates rows already solved)."""
+    m, n = 40, 8
+    say(f"dtrsm Inf in B uplo={uplo:#x}")
+    a, _ = tri_matrix(m, uplo, False, np.float64, seed=9)
+    b = Mat(m, n, seed=10)
+    bv = b.view()
+    bv[17, 2] = np.inf
+    bv[5, 6] = -np.inf
+    b0 = bv.copy()
+    a.to_device()
+    b.to_device()
+    run_trsm(blis, np.float64, 0, uplo, "n", 0, m, n, 1.0, a, b)
+    got = b.device_view()
+    ref = b0.copy(order="F")
+    oracle.trsm(0, uplo, "n", 0, 1.0, a.view().copy(order="F"), ref)
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(ref))
+    np.testing.assert_array_equal(np.isinf(got), np.isinf(ref))
+    fin = np.isfinite(ref)
+    np.testing.assert_allclose(got[fin], ref[fin], rtol=1e-12, atol=1e-12)

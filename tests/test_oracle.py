@@ -31,7 +31,8 @@ NETLIB_IN = ROOT / "tests" / "golden" / "netlib"
 
 
 def _ensure_ref_built():
-    if not all((REF_DIR / f"{d}_oracle.x").exists() for d in ("dblat3", "dblat2", "sblat3", "sblat2")):
+    drivers = ("dblat3", "dblat2", "sblat3", "sblat2", "dblat1", "sblat1")
+    if not all((REF_DIR / f"{d}_oracle.x").exists() for d in drivers):
         if not Path("/root/reference").exists():
             pytest.skip("netlib drivers not built and the reference tree is absent")
         subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "all", "ref"], check=True)
@@ -39,8 +40,10 @@ def _ensure_ref_built():
 
 def run_netlib(driver: str, suffix: str) -> str:
     exe = REF_DIR / f"{driver}_{suffix}.x"
+    deck = NETLIB_IN / f"{driver}.in"
     with tempfile.TemporaryDirectory() as td:
-        with open(NETLIB_IN / f"{driver}.in") as fin:
+        # The level-1 drivers take no input deck.
+        with open(deck if deck.exists() else os.devnull) as fin:
             p = subprocess.run([str(exe)], stdin=fin, cwd=td, capture_output=True, text=True,
                                timeout=600)
         out_file = Path(td) / f"out.{driver}"
@@ -70,6 +73,31 @@ def check_netlib_output(text: str, routines, calls):
 def test_netlib_drivers_on_oracle(driver, routines, calls):
     _ensure_ref_built()
     check_netlib_output(run_netlib(driver, "oracle"), routines, calls)
+
+
+# Level-1 drivers: thirteen routines each, one "Test of subprogram number"
+# section per routine ending in "----- PASS -----" when every check held.
+NETLIB1_CASES = [
+    ("dblat1", ["DDOT", "DAXPY", "DROTG", "DROT", "DCOPY", "DSWAP", "DNRM2", "DASUM", "DSCAL",
+                "IDAMAX", "DROTMG", "DROTM", "DSDOT"]),
+    ("sblat1", ["SDOT", "SAXPY", "SROTG", "SROT", "SCOPY", "SSWAP", "SNRM2", "SASUM", "SSCAL",
+                "ISAMAX", "SROTMG", "SROTM", "SDSDOT"]),
+]
+
+
+def check_netlib1_output(text: str, routines):
+    parts = text.split("Test of subprogram number")[1:]
+    assert len(parts) == len(routines), text[-3000:]
+    for part, name in zip(parts, routines):
+        head = part.splitlines()[0].split()
+        assert head[-1] == name, (head, name)
+        assert "----- PASS -----" in part and "FAIL" not in part, part[:2000]
+
+
+@pytest.mark.parametrize("driver,routines", NETLIB1_CASES)
+def test_netlib1_drivers_on_oracle(driver, routines):
+    _ensure_ref_built()
+    check_netlib1_output(run_netlib(driver, "oracle"), routines)
 
 
 # ---------------------------------------------------------------- GEMM --
@@ -323,3 +351,60 @@ def test_double_double_gemm_is_correctly_rounded(oracle):
             want = float(exact)          # correctly rounded
             assert abs(got[i, j] - want) <= np.spacing(abs(want)), (i, j, got[i, j], want)
     assert np.all(got[:, 0] == beta * c0[:, 0])
+
+
+# ---------------------------------------------------------- gemm_batch --
+
+@pytest.mark.parametrize("ch", ["d", "s"])
+def test_oracle_gemm_batch_is_a_loop_of_gemm(oracle, ch):
+    """oracle_?gemm_batch follows the group semantics of ?gemm_batch_
+    (frame/compat/extra/bla_gemm_batch.c:172-241): group g's parameters apply
+    to its group_size[g] problems, whose pointers follow one another in the
+    arrays (an empty group consumes none). Pinned by bitwise equality with
+    one oracle gemm call per problem."""
+    import ctypes as C
+    dt = np.float64 if ch == "d" else np.float32
+    T = C.c_double if ch == "d" else C.c_float
+    rng = np.random.default_rng(3)
+    groups = [(2, "n", "t", 7, 5, 9, 1.5, 0.0), (0, "t", "t", 3, 3, 3, 1.0, 1.0),
+              (3, "t", "n", 12, 4, 6, -0.5, 0.25), (1, "n", "n", 1, 8, 2, 2.0, -1.0)]
+    probs = []
+    for (size, ta, tb, m, n, k, al, be) in groups:
+        for _ in range(size):
+            a = np.asfortranarray(rng.uniform(-1, 1, (k, m) if ta == "t" else (m, k)).astype(dt))
+            b = np.asfortranarray(rng.uniform(-1, 1, (n, k) if tb == "t" else (k, n)).astype(dt))
+            c = np.asfortranarray(rng.uniform(-1, 1, (m, n)).astype(dt))
+            probs.append((a, b, c))
+    G = len(groups)
+    TRV = {"n": 0x0, "t": 0x8}
+    I = C.c_int64 * G
+    ta = (C.c_int * G)(*[TRV[g[1]] for g in groups])
+    tb = (C.c_int * G)(*[TRV[g[2]] for g in groups])
+    m, n, k = (I(*[g[i] for g in groups]) for i in (3, 4, 5))
+    alpha = (T * G)(*[g[6] for g in groups])
+    beta = (T * G)(*[g[7] for g in groups])
+    lda = I(*[(g[5] if g[1] == "t" else g[3]) for g in groups])
+    ldb = I(*[(g[4] if g[2] == "t" else g[5]) for g in groups])
+    ldc = I(*[g[3] for g in groups])
+    gs = I(*[g[0] for g in groups])
+    P = C.c_void_p * len(probs)
+    outs = [p[2].copy(order="F") for p in probs]
+    aptr = P(*[p[0].ctypes.data for p in probs])
+    bptr = P(*[p[1].ctypes.data for p in probs])
+    cptr = P(*[o.ctypes.data for o in outs])
+    fn = getattr(oracle.lib(), f"oracle_{ch}gemm_batch")
+    fn.restype = None
+    fn.argtypes = [C.c_void_p] * 13 + [C.c_int64, C.c_void_p]
+    fn(C.cast(ta, C.c_void_p), C.cast(tb, C.c_void_p), C.cast(m, C.c_void_p),
+       C.cast(n, C.c_void_p), C.cast(k, C.c_void_p), C.cast(alpha, C.c_void_p),
+       C.cast(aptr, C.c_void_p), C.cast(lda, C.c_void_p), C.cast(bptr, C.c_void_p),
+       C.cast(ldb, C.c_void_p), C.cast(beta, C.c_void_p), C.cast(cptr, C.c_void_p),
+       C.cast(ldc, C.c_void_p), G, C.cast(gs, C.c_void_p))
+    idx = 0
+    for (size, gta, gtb, gm, gn, gk, al, be) in groups:
+        for _ in range(size):
+            a, b, c = probs[idx]
+            ref = c.copy(order="F")
+            oracle.gemm(gta, gtb, al, a, b, be, ref)
+            np.testing.assert_array_equal(outs[idx], ref)
+            idx += 1
