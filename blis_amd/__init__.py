@@ -33,6 +33,7 @@ BLIS_NONUNIT_DIAG = 0x0
 BLIS_UNIT_DIAG = 0x100
 BLIS_FLOAT = 0
 BLIS_DOUBLE = 2
+BLIS_SYMMETRIC = 0x2 << 20
 BLIS_TRIANGULAR = 0x3 << 20
 
 dim_t = C.c_int64
@@ -104,6 +105,24 @@ def _sig(lib):
         s(f"{ch}axpy_", None, PF, PT, P, PF, P, PF)
         s(f"{ch}dot_", T, PF, P, PF, P, PF)
         s(f"{ch}gemm_batch_", None, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P)
+        # structured level-3 (typed and BLAS)
+        s(f"bli_{ch}gemmt", None, C.c_int, C.c_int, C.c_int, dim_t, dim_t, PT, P, inc_t, inc_t,
+          P, inc_t, inc_t, PT, P, inc_t, inc_t)
+        s(f"bli_{ch}syrk", None, C.c_int, C.c_int, dim_t, dim_t, PT, P, inc_t, inc_t, PT, P,
+          inc_t, inc_t)
+        s(f"bli_{ch}syr2k", None, C.c_int, C.c_int, C.c_int, dim_t, dim_t, PT, P, inc_t, inc_t,
+          P, inc_t, inc_t, PT, P, inc_t, inc_t)
+        s(f"bli_{ch}symm", None, C.c_int, C.c_int, C.c_int, C.c_int, dim_t, dim_t, PT, P, inc_t,
+          inc_t, P, inc_t, inc_t, PT, P, inc_t, inc_t)
+        s(f"bli_{ch}trmm", None, C.c_int, C.c_int, C.c_int, C.c_int, dim_t, dim_t, PT, P, inc_t,
+          inc_t, P, inc_t, inc_t)
+        s(f"bli_{ch}trmm3", None, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dim_t, dim_t, PT,
+          P, inc_t, inc_t, P, inc_t, inc_t, PT, P, inc_t, inc_t)
+        s(f"{ch}symm_", None, PC, PC, PF, PF, PT, P, PF, P, PF, PT, P, PF)
+        s(f"{ch}trmm_", None, PC, PC, PC, PC, PF, PF, PT, P, PF, P, PF)
+        s(f"{ch}syrk_", None, PC, PC, PF, PF, PT, P, PF, PT, P, PF)
+        s(f"{ch}syr2k_", None, PC, PC, PF, PF, PT, P, PF, P, PF, PT, P, PF)
+        s(f"{ch}gemmt_", None, PC, PC, PC, PF, PF, PT, P, PF, P, PF, PT, P, PF)
         s(f"bli_amd_{ch}gemm_split", C.c_int, C.c_int, C.c_int, dim_t, dim_t, dim_t, PT, P,
           inc_t, inc_t, P, inc_t, inc_t, PT, P, inc_t, inc_t, C.c_int, C.POINTER(C.c_double))
     s("bli_amd_gemm_batch_plan_execute", None, P, C.c_int)
@@ -120,6 +139,12 @@ def _sig(lib):
     s("bli_info_get_version_str", C.c_char_p)
     s("bli_gemm", None, P, P, P, P, P)
     s("bli_trsm", None, C.c_int, P, P, P)
+    s("bli_gemmt", None, P, P, P, P, P)
+    s("bli_syrk", None, P, P, P, P)
+    s("bli_syr2k", None, P, P, P, P, P)
+    s("bli_symm", None, C.c_int, P, P, P, P, P)
+    s("bli_trmm", None, C.c_int, P, P, P)
+    s("bli_trmm3", None, C.c_int, P, P, P, P, P)
     s("bli_obj_create_with_attached_buffer", None, C.c_int, dim_t, dim_t, P, inc_t, inc_t, P)
     s("bli_obj_create_1x1_with_attached_buffer", None, C.c_int, P, P)
     if hasattr(lib, "bli_amd_gemm_set_config"):

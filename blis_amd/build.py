@@ -49,6 +49,11 @@ SOURCES = [
     "split.hip",
     "mlp.hip",
     "select.hip",
+    "struc.hip",
+    "api_struc.cpp",
+    "complex.hip",
+    "api_complex.cpp",
+    "cblas.cpp",
 ]
 
 

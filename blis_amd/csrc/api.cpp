@@ -19,7 +19,7 @@
 //   negative BLAS increments    frame/include/bli_blas_macro_defs.h:50-74
 //   gemm_batch group indexing   frame/compat/extra/bla_gemm_batch.c:135-246
 //   xerbla message              frame/compat/f2c/bla_xerbla.c
-#include "internal.hpp"
+#include "api_common.hpp"
 
 #include <cctype>
 #include <cstring>
@@ -27,22 +27,22 @@
 #include <set>
 
 using namespace blis_amd;
+using namespace blis_amd::api;
 
 #define BLIS_EXPORT extern "C" __attribute__((visibility("default")))
 
+namespace blis_amd {
+namespace api {
+errlev_t& errlev() {
+    static errlev_t level = BLIS_FULL_ERROR_CHECKING;
+    return level;
+}
+} // namespace api
+} // namespace blis_amd
+
 namespace {
 
-// Layout of the buffer behind BLIS's global scalar constants
-// (frame/include/bli_type_defs.h constdata_t): one value per datatype.
-struct ConstData {
-    float    s;
-    double   d;
-    scomplex c;
-    dcomplex z;
-    gint_t   i;
-};
-
-ConstData k_two   = { 2.0f,  2.0, { 2.0f, 0.0f }, { 2.0, 0.0 },  2 };
+ConstData k_two  = { 2.0f,  2.0, { 2.0f, 0.0f }, { 2.0, 0.0 },  2 };
 ConstData k_one   = { 1.0f,  1.0, { 1.0f, 0.0f }, { 1.0, 0.0 },  1 };
 ConstData k_zero  = { 0.0f,  0.0, { 0.0f, 0.0f }, { 0.0, 0.0 },  0 };
 ConstData k_mone  = {-1.0f, -1.0, {-1.0f, 0.0f }, {-1.0, 0.0 }, -1 };
@@ -57,37 +57,9 @@ obj_t make_const(ConstData* p) {
     return o;
 }
 
-errlev_t g_errlev = BLIS_FULL_ERROR_CHECKING;
 
 std::mutex g_alloc_mutex;
 std::set<void*> g_owned;   // buffers allocated by bli_obj_create
-
-inline bool check_on() { return g_errlev == BLIS_FULL_ERROR_CHECKING; }
-
-inline void require(bool ok, err_t code, const char* where) {
-    if (!ok && check_on()) abort_with(code, where);
-}
-
-inline bool valid_trans(trans_t t) {
-    return t == BLIS_NO_TRANSPOSE || t == BLIS_TRANSPOSE ||
-           t == BLIS_CONJ_NO_TRANSPOSE || t == BLIS_CONJ_TRANSPOSE;
-}
-inline bool does_trans(trans_t t) { return ( (unsigned)t & BLIS_TRANS_BIT ) != 0; }
-
-template <typename T> constexpr Dt dt_for();
-template <> constexpr Dt dt_for<double>() { return Dt::F64; }
-template <> constexpr Dt dt_for<float>() { return Dt::F32; }
-
-// Read a scalar held by a host or device pointer.
-template <typename T>
-T read_scalar(const T* p) {
-    if (is_device_pointer(p)) {
-        T v;
-        HIP_CHECK(hipMemcpy(&v, p, sizeof(T), hipMemcpyDeviceToHost));
-        return v;
-    }
-    return *p;
-}
 
 // ---- level-3 cores ------------------------------------------------------
 
@@ -273,59 +245,7 @@ void gemv_core(trans_t transa, dim_t m, dim_t n, const T* alpha,
 
 // ---- obj_t helpers --------------------------------------------------------
 
-inline bool obj_is_real_fp(const obj_t* o) {
-    const num_t dt = bli_obj_dt(o);
-    return dt == BLIS_FLOAT || dt == BLIS_DOUBLE;
-}
-
-// Value of a 1x1 scalar object in double precision.
-double obj_scalar(const obj_t* o) {
-    require(o != nullptr, BLIS_NULL_POINTER, "scalar object");
-    const num_t dt = bli_obj_dt(o);
-    const char* base = (const char*)o->buffer;
-    require(base != nullptr, BLIS_EXPECTED_NONNULL_OBJECT_BUFFER, "scalar object buffer");
-    if (dt == BLIS_CONSTANT) return ((const ConstData*)base)->d;
-    const char* p = base + (o->off[0] * o->rs + o->off[1] * o->cs) * (inc_t)o->elem_size;
-    if (dt == BLIS_DOUBLE) return read_scalar((const double*)p);
-    if (dt == BLIS_FLOAT) return (double)read_scalar((const float*)p);
-    abort_with(BLIS_EXPECTED_REAL_DATATYPE, "scalar object datatype");
-    return 0.0;
-}
-
-inline void* obj_origin(const obj_t* o) {
-    return (char*)o->buffer + (o->off[0] * o->rs + o->off[1] * o->cs) * (inc_t)o->elem_size;
-}
-
-inline dim_t obj_vec_dim(const obj_t* o) { return o->dim[0] == 1 ? o->dim[1] : o->dim[0]; }
-inline inc_t obj_vec_inc(const obj_t* o) { return o->dim[0] == 1 ? o->cs : o->rs; }
-
-void check_matrix_obj(const obj_t* o, const char* where) {
-    require(o != nullptr, BLIS_NULL_POINTER, where);
-    require(obj_is_real_fp(o), BLIS_EXPECTED_REAL_DATATYPE, where);
-    require(o->dim[0] >= 0 && o->dim[1] >= 0, BLIS_NEGATIVE_DIMENSION, where);
-    require(o->buffer != nullptr || o->dim[0] == 0 || o->dim[1] == 0,
-            BLIS_EXPECTED_NONNULL_OBJECT_BUFFER, where);
-}
-
 // ---- BLAS helpers --------------------------------------------------------
-
-inline char up(char ch) { return (char)std::toupper((unsigned char)ch); }
-
-bool map_trans(char ch, trans_t* t) {
-    switch (up(ch)) {
-    case 'N': *t = BLIS_NO_TRANSPOSE; return true;
-    case 'T': *t = BLIS_TRANSPOSE; return true;
-    case 'C': *t = BLIS_CONJ_TRANSPOSE; return true;
-    default: return false;
-    }
-}
-
-void blas_error(const char* dt, const char* op, f77_int info) {
-    char name[16];
-    std::snprintf(name, sizeof(name), "%s%-5s", dt, op);
-    for (char* q = name; *q; ++q) *q = up(*q);
-    xerbla_(name, &info, (ftnlen)6);
-}
 
 template <typename T>
 void blas_gemm(const char* dtc, const f77_char* transa, const f77_char* transb,
@@ -496,8 +416,8 @@ BLIS_EXPORT obj_t BLIS_MINUS_TWO = make_const(&k_mtwo);
 BLIS_EXPORT void bli_init(void) { init_once(); }
 BLIS_EXPORT void bli_finalize(void) {}
 BLIS_EXPORT const char* bli_info_get_version_str(void) { return BLIS_VERSION_STRING; }
-BLIS_EXPORT void bli_error_checking_level_set(errlev_t l) { g_errlev = l; }
-BLIS_EXPORT errlev_t bli_error_checking_level(void) { return g_errlev; }
+BLIS_EXPORT void bli_error_checking_level_set(errlev_t l) { errlev() = l; }
+BLIS_EXPORT errlev_t bli_error_checking_level(void) { return errlev(); }
 BLIS_EXPORT bool bli_error_checking_is_enabled(void) { return check_on(); }
 
 BLIS_EXPORT const char* bli_error_string_for_code(gint_t code) {
@@ -539,6 +459,10 @@ BLIS_EXPORT void bli_obj_create_without_buffer(num_t dt, dim_t m, dim_t n, obj_t
     o.root = obj;
     o.dim[0] = m; o.dim[1] = n;
     o.info = ( o.info & ~BLIS_DATATYPE_BITS ) | (objbits_t)dt;
+    // The computation precision defaults to the storage precision
+    // (frame/base/bli_obj.c:111).
+    if (dt == BLIS_FLOAT || dt == BLIS_DOUBLE || dt == BLIS_SCOMPLEX || dt == BLIS_DCOMPLEX)
+        o.info |= ( (objbits_t)dt & BLIS_PRECISION_BIT ) << BLIS_COMP_PREC_SHIFT;
     o.info2 = ( (objbits_t)dt << BLIS_SCALAR_DT_SHIFT ) & BLIS_SCALAR_DT_BITS;
     o.elem_size = (dt == BLIS_DOUBLE) ? 8 : (dt == BLIS_FLOAT) ? 4 :
                   (dt == BLIS_SCOMPLEX) ? 8 : (dt == BLIS_DCOMPLEX) ? 16 : 8;
@@ -682,11 +606,20 @@ BLIS_EXPORT void bli_gemm_ex(const obj_t* alpha, const obj_t* a, const obj_t* b,
                              const cntx_t* cntx, const rntm_t* rntm) {
     (void)cntx; (void)rntm;
     init_once();
+    require(a != nullptr && b != nullptr && c != nullptr, BLIS_NULL_POINTER, "bli_gemm");
+    {
+        // Complex operands, mixed datatypes, or a computation precision other
+        // than C's storage precision take the mixed-datatype path.
+        const num_t dc = bli_obj_dt(c);
+        const unsigned cp = (c->info & BLIS_COMP_PREC_BIT) >> BLIS_COMP_PREC_SHIFT;
+        const bool plain = (dc == BLIS_FLOAT || dc == BLIS_DOUBLE) &&
+                           bli_obj_dt(a) == dc && bli_obj_dt(b) == dc &&
+                           cp == ((unsigned)dc & BLIS_PRECISION_BIT);
+        if (!plain) { gemm_obj_md(alpha, a, b, beta, c); return; }
+    }
     check_matrix_obj(a, "bli_gemm: a");
     check_matrix_obj(b, "bli_gemm: b");
     check_matrix_obj(c, "bli_gemm: c");
-    require(bli_obj_dt(a) == bli_obj_dt(c) && bli_obj_dt(b) == bli_obj_dt(c),
-            BLIS_INCONSISTENT_DATATYPES, "bli_gemm");
     const dim_t m = c->dim[0], n = c->dim[1], k = bli_obj_width_after_trans(a);
     require(bli_obj_length_after_trans(a) == m && bli_obj_width_after_trans(b) == n &&
             bli_obj_length_after_trans(b) == k, BLIS_NONCONFORMAL_DIMENSIONS, "bli_gemm");

@@ -473,6 +473,73 @@ __device__ inline void store_tile(const TileProblem<T>& p,
     }
 }
 
+// Stored region of a structured C (gemmt, syrk, syr2k): element (i, j) is
+// written only when it lies on the stored side of the diagonal j - i == doff
+// (lower: j - i <= doff; upper: j - i >= doff), BLIS's convention for
+// diagonal offsets (frame/include/bli_param_macro_defs.h:638-673).
+// uplo: 0 = every element, 1 = lower, 2 = upper.
+struct CMask {
+    int     uplo = 0;
+    int64_t doff = 0;
+};
+
+// store_tile restricted to the stored region of C: a TM-row vector is
+// written whole when all its rows are stored, otherwise element by element.
+// The per-element arithmetic is the same epilogue() as store_tile.
+template <typename T, class TL>
+__device__ inline void store_tile_masked(const TileProblem<T>& p,
+                                         const typename Mma<T>::acc_t (&acc)[TL::TM][TL::TN],
+                                         int64_t row0, int64_t col0, int lane, CMask cm) {
+    using vec_t = typename Mma<T>::vec_t;
+    constexpr int V = TL::V;
+    constexpr int TM = TL::TM;
+    const int g = lane >> 4, lc = lane & 15;
+    const T alpha = p.alpha, beta = p.beta;
+    const bool lower = cm.uplo == 1;
+#pragma unroll
+    for (int j = 0; j < TL::TN; ++j) {
+        const int64_t col = col0 + TL::TN * lc + j;
+        if (col >= p.n) continue;
+        // lower: rows i >= lim are stored; upper: rows i <= lim.
+        const int64_t lim = col - cm.doff;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t rb = row0 + (int64_t)TM * Mma<T>::acc_row(r, g);
+            if constexpr (TM % V == 0) {
+                const bool all_in = lower ? rb >= lim : rb + TM - 1 <= lim;
+                if (all_in && p.vec_c && rb + TM <= p.m) {
+                    T* dst = p.c + rb + col * p.c_sj;
+#pragma unroll
+                    for (int q = 0; q < TM / V; ++q) {
+                        vec_t o;
+                        if (beta == T(0)) {
+#pragma unroll
+                            for (int e = 0; e < V; ++e) o[e] = alpha * acc[q * V + e][j][r];
+                        } else {
+                            const vec_t old = *reinterpret_cast<const vec_t*>(dst + q * V);
+#pragma unroll
+                            for (int e = 0; e < V; ++e)
+                                o[e] = epilogue(alpha, acc[q * V + e][j][r], beta, old[e]);
+                        }
+                        *reinterpret_cast<vec_t*>(dst + q * V) = o;
+                    }
+                    continue;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < TM; ++i) {
+                const int64_t row = rb + i;
+                const bool in = lower ? row >= lim : row <= lim;
+                if (row < p.m && in) {
+                    T* dst = p.c + row * p.c_si + col * p.c_sj;
+                    *dst = (beta == T(0)) ? alpha * acc[i][j][r]
+                                          : epilogue(alpha, acc[i][j][r], beta, *dst);
+                }
+            }
+        }
+    }
+}
+
 // Epilogue through LDS, for tiles whose wave rows per lane are too few for
 // full-width vector access to C: the accumulators are written to LDS as a
 // column-major BM x BN tile (column pitch BM + V), then each thread updates
@@ -569,9 +636,9 @@ __device__ inline void prefetch_c(const TileProblem<T>& p, int64_t m0, int64_t n
 }
 
 template <typename T, class TL, bool A_K, bool B_K, bool FAST, int PF, bool EDGE, bool LDS_EPI,
-          bool C_PF = false>
+          bool C_PF = false, bool MASK = false>
 __device__ inline void gemm_tile_impl(const TileProblem<T>& p, int64_t m0, int64_t n0, T* smem,
-                                      int* sink = nullptr) {
+                                      int* sink = nullptr, CMask cm = CMask{}) {
     static_assert(PF == 1 || PF == 2, "prefetch depth");
     using acc_t = typename Mma<T>::acc_t;
     const int tid = threadIdx.x;
@@ -685,7 +752,9 @@ __device__ inline void gemm_tile_impl(const TileProblem<T>& p, int64_t m0, int64
         }
     }
 
-    if constexpr (LDS_EPI)
+    if constexpr (MASK)
+        store_tile_masked<T, TL>(p, acc, m0 + wm0, n0 + wn0, lane, cm);
+    else if constexpr (LDS_EPI)
         store_tile_lds<T, TL>(p, acc, m0, n0, wm0, wn0, smem);
     else if (FAST && p.vec_c)
         store_tile<T, TL, true>(p, acc, m0 + wm0, n0 + wn0, lane);
@@ -696,18 +765,22 @@ __device__ inline void gemm_tile_impl(const TileProblem<T>& p, int64_t m0, int64
 // LDS_EPI: write C through store_tile_lds (the batch kernels) instead of
 // straight from the accumulators. C_PF: prefetch the C tile into L2 one
 // slab before the epilogue (one-slab prefetch depth only), through `sink`
-// (NT ints of LDS nobody reads).
+// (NT ints of LDS nobody reads). MASK: write only the stored region `cm` of
+// a structured C (store_tile_masked).
 template <typename T, class TL, bool A_K, bool B_K, int PF = 1, bool EDGE = false,
-          bool LDS_EPI = false, bool C_PF = false>
+          bool LDS_EPI = false, bool C_PF = false, bool MASK = false>
 __device__ inline void gemm_tile(const TileProblem<T>& p, int64_t tm, int64_t tn, T* smem,
-                                 int* sink = nullptr) {
+                                 int* sink = nullptr, CMask cm = CMask{}) {
     static_assert(!C_PF || PF == 1, "C prefetch is placed for one slab in flight");
+    static_assert(!MASK || !LDS_EPI, "masked stores go straight from the accumulators");
     const int64_t m0 = tm * TL::BM, n0 = tn * TL::BN;
     const bool fast = p.vec_a && p.vec_b && m0 + TL::BM <= p.m && n0 + TL::BN <= p.n;
     if (fast)
-        gemm_tile_impl<T, TL, A_K, B_K, true, PF, EDGE, LDS_EPI, C_PF>(p, m0, n0, smem, sink);
+        gemm_tile_impl<T, TL, A_K, B_K, true, PF, EDGE, LDS_EPI, C_PF, MASK>(p, m0, n0, smem, sink,
+                                                                            cm);
     else
-        gemm_tile_impl<T, TL, A_K, B_K, false, PF, EDGE, LDS_EPI, C_PF>(p, m0, n0, smem, sink);
+        gemm_tile_impl<T, TL, A_K, B_K, false, PF, EDGE, LDS_EPI, C_PF, MASK>(p, m0, n0, smem,
+                                                                             sink, cm);
 }
 
 } // namespace dev

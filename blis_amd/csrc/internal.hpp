@@ -57,8 +57,11 @@ BLIS_AMD_HIDDEN void        sync_if_needed();
 // Grow-only device scratch, one buffer per slot: 0-1 GEMM operand packing
 // and transposes, 2-5 split-precision slices and statistics, 5-7 level-1/2
 // partials, 8-10 autotuner operands, 11-12 batch autotuner, 13-16 TRSM,
-// 17 GEMM split-k partitions, 18-19 TRSM 512-row inverses.
-constexpr int WS_SLOTS = 20;
+// 17 GEMM split-k partitions, 18-19 TRSM 512-row inverses, 20-21 dense
+// copies of structured operands (symm/trmm) and trmm's copy of B, 22-24
+// complex GEMM (A in 1e format, B in 1r format, a column-major copy of C),
+// 25-27 mixed-datatype GEMM (A and B in the computation type, the product).
+constexpr int WS_SLOTS = 28;
 BLIS_AMD_HIDDEN void*       workspace(size_t bytes, int slot);
 BLIS_AMD_HIDDEN bool        synchronous();   // calls wait for their results
 BLIS_AMD_HIDDEN bool        is_device_pointer(const void* p);
@@ -87,6 +90,9 @@ BLIS_AMD_HIDDEN void stage(Staged& s, const void* p, Dt dt, int64_t m, int64_t n
                            int64_t rs, int64_t cs, bool is_output);
 BLIS_AMD_HIDDEN void stage_vec(Staged& s, const void* p, Dt dt, int64_t n,
                                int64_t inc, bool is_output);
+// A complex matrix (real precision dt, strides in complex elements).
+BLIS_AMD_HIDDEN void stage_complex(Staged& s, const void* p, Dt dt, int64_t m, int64_t n,
+                                   int64_t rs, int64_t cs, bool is_output);
 
 // ---- kernels' launchers ----
 BLIS_AMD_HIDDEN void gemm_run(const GemmDesc& d, const GemmConfig* cfg, hipStream_t st);
@@ -95,6 +101,24 @@ BLIS_AMD_HIDDEN void gemm_run(const GemmDesc& d, const GemmConfig* cfg, hipStrea
 BLIS_AMD_HIDDEN void gemm_strided_run(const GemmDesc& d, int count, int64_t sa, int64_t sb,
                                       int64_t sc, hipStream_t st);
 BLIS_AMD_HIDDEN void scalm_run(Dt dt, int64_t m, int64_t n, double beta, View c, hipStream_t st);
+
+// ---- structured level-3 (struc.hip) ----
+// gemmt: C := beta*C + alpha*A_eff*B_eff on the uploc triangle of the m x m C
+// (diagonal offset doff); syrk/syr2k are built from it.
+BLIS_AMD_HIDDEN void gemmt_run(const GemmDesc& d, uplo_t uploc, int64_t doff, hipStream_t st);
+// symm: C := beta*C + alpha*A*B (left) or alpha*B*A (right), A symmetric and
+// stored in its uploa triangle.
+BLIS_AMD_HIDDEN void symm_run(Dt dt, side_t side, uplo_t uploa, int64_t m, int64_t n,
+                              double alpha, View a, View b, double beta, View c, hipStream_t st);
+// trmm3: C := beta*C + alpha*op(A)*B_eff (left) or alpha*B_eff*op(A) (right),
+// A triangular (uploa, diaga, diagonal offset doff); B_eff has transb folded in.
+BLIS_AMD_HIDDEN void trmm3_run(Dt dt, side_t side, uplo_t uploa, trans_t transa, diag_t diaga,
+                               int64_t doff, int64_t m, int64_t n, double alpha, View a, View b,
+                               double beta, View c, hipStream_t st);
+// trmm: B := alpha*op(A)*B (left) or alpha*B*op(A) (right), in place.
+BLIS_AMD_HIDDEN void trmm_run(Dt dt, side_t side, uplo_t uploa, trans_t transa, diag_t diaga,
+                              int64_t doff, int64_t m, int64_t n, double alpha, View a, View b,
+                              hipStream_t st);
 BLIS_AMD_HIDDEN void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag,
                               int64_t m, int64_t n, double alpha, View a, View b,
                               hipStream_t st);
@@ -115,6 +139,35 @@ struct BatchProblem {
 };
 BLIS_AMD_HIDDEN void gemm_batch_run(Dt dt, const std::vector<BatchProblem>& probs,
                                     hipStream_t st);
+
+// ---- complex and mixed-datatype GEMM (complex.hip) ----
+// C := beta*C + alpha*op(A)*op(B) for complex operands of real precision dt.
+// Views are in complex elements with transposition folded into the strides;
+// conja/conjb conjugate the operand.
+struct ZGemmDesc {
+    Dt      dt = Dt::F64;
+    int64_t m = 0, n = 0, k = 0;
+    View    a, b, c;
+    bool    conja = false, conjb = false;
+    double  alpha_re = 1.0, alpha_im = 0.0, beta_re = 0.0, beta_im = 0.0;
+};
+BLIS_AMD_HIDDEN void zgemm_run(const ZGemmDesc& z, hipStream_t st);
+BLIS_AMD_HIDDEN void zscalm_run(Dt dt, int64_t m, int64_t n, double br, double bi, View c,
+                                hipStream_t st);
+// Mixed-datatype element conversion: dst (column-major, ld ldd, complex
+// when dst_cx) := src (strided, complex when src_cx); a complex source going
+// to a real destination gives its real part, a real source going to a
+// complex destination gets a zero imaginary part.
+BLIS_AMD_HIDDEN void md_convert_run(Dt sdt, bool src_cx, const void* src, int64_t m, int64_t n,
+                                    int64_t rs, int64_t cs, Dt ddt, bool dst_cx, void* dst,
+                                    int64_t ldd, hipStream_t st);
+// C := beta*C + P for a product P (column-major, ld ldp) in another datatype.
+// Complex C and P: complex beta. Complex C, real P: the real parts are
+// updated with real(beta), imaginary parts untouched. Real C, complex P: the
+// real part of P is added. beta == 0 does not read C.
+BLIS_AMD_HIDDEN void md_accum_run(Dt pdt, bool p_cx, const void* p, int64_t ldp, int64_t m,
+                                  int64_t n, double br, double bi, Dt cdt, bool c_cx, View c,
+                                  hipStream_t st);
 
 // ---- configuration selection (select.hip) ----
 enum { SELECT_HEURISTIC = 0, SELECT_LEARNED = 1, SELECT_EXHAUSTIVE = 2 };

@@ -174,6 +174,15 @@ void stage(Staged& s, const void* p, Dt dt, int64_t m, int64_t n, int64_t rs, in
     stage_span(s, p, dt, lo, hi, is_output);
 }
 
+void stage_complex(Staged& s, const void* p, Dt dt, int64_t m, int64_t n, int64_t rs,
+                   int64_t cs, bool is_output) {
+    if (m <= 0 || n <= 0) { s.dev = const_cast<void*>(p); return; }
+    int64_t lo, hi;
+    span_of(m, n, rs, cs, lo, hi);
+    // In reals: element e spans [2e, 2e + 1].
+    stage_span(s, p, dt, 2 * lo, 2 * hi + 1, is_output);
+}
+
 void stage_vec(Staged& s, const void* p, Dt dt, int64_t n, int64_t inc, bool is_output) {
     if (n <= 0) { s.dev = const_cast<void*>(p); return; }
     int64_t lo, hi;

@@ -16,6 +16,8 @@
  *   object level-3        frame/3/bli_l3_oapi_ex.c:48-148 (gemm), :692-801 (trsm)
  *   level-1v typed        frame/1/bli_l1v_tapi.c:155-191 (axpyv), :195-232 (dotv)
  *   level-2 typed         frame/2/bli_l2_tapi.c:43-123 (gemv)
+ *   structured level-3    frame/3/bli_l3_tapi_ex.h:60-225, bli_l3_oapi_ex.c:151-690
+ *   complex / mixed GEMM  frame/3/bli_l3_ind.c (1m), docs/MixedDatatypes.md
  *   BLAS compatibility    frame/compat/bla_gemm.c:122-260, bla_trsm.c:121-216,
  *                         bla_gemv.c:41-141, bla_axpy.c:41-85, bla_dot.c:41-88,
  *                         extra/bla_gemm_batch.c:135-246
@@ -195,6 +197,11 @@ typedef enum num_e {
     BLIS_DT_HI    = BLIS_DCOMPLEX
 } num_t;
 
+typedef enum prec_e {
+    BLIS_SINGLE_PREC = BLIS_BITVAL_SINGLE_PREC,
+    BLIS_DOUBLE_PREC = BLIS_BITVAL_DOUBLE_PREC
+} prec_t;
+
 /* Error codes (frame/include/bli_type_defs.h:1500-1608; same values). */
 typedef enum err_e {
     BLIS_SUCCESS                              = (  -1),
@@ -355,6 +362,13 @@ static inline void bli_obj_set_struc( struc_t s, obj_t* o )
 { o->info = ( objbits_t )( ( o->info & ~BLIS_STRUC_BITS ) | ( objbits_t )s ); }
 static inline void bli_obj_set_diag_offset( doff_t d, obj_t* o ) { o->diag_off = d; }
 static inline void bli_obj_toggle_trans( obj_t* o ) { o->info ^= BLIS_TRANS_BIT; }
+/* Computation precision of a (mixed-datatype) GEMM is read from C
+   (frame/include/bli_obj_macro_defs.h:149-153, 401-406). */
+static inline prec_t bli_obj_comp_prec( const obj_t* o )
+{ return ( prec_t )( ( o->info & BLIS_COMP_PREC_BIT ) >> BLIS_COMP_PREC_SHIFT ); }
+static inline void bli_obj_set_comp_prec( prec_t p, obj_t* o )
+{ o->info = ( objbits_t )( ( o->info & ~BLIS_COMP_PREC_BIT ) | ( ( objbits_t )p << BLIS_COMP_PREC_SHIFT ) ); }
+static inline bool bli_obj_is_complex( const obj_t* o ) { return ( o->info & BLIS_DOMAIN_BIT ) != 0; }
 
 /* ---- initialization / information (frame/base/bli_init.c:40-97) ------- */
 
@@ -420,6 +434,91 @@ void bli_dtrsm_ex( side_t side, uplo_t uploa, trans_t transa, diag_t diaga,
                    double* b, inc_t rs_b, inc_t cs_b,
                    const cntx_t* cntx, const rntm_t* rntm );
 
+/* Complex GEMM, typed (frame/3/bli_l3_tapi_ex.c:43-99, scomplex/dcomplex
+   instances). trans_t may conjugate: BLIS_CONJ_NO_TRANSPOSE and
+   BLIS_CONJ_TRANSPOSE. Computed by the 1m method on the real GEMM tiles. */
+void bli_cgemm( trans_t transa, trans_t transb, dim_t m, dim_t n, dim_t k,
+                const scomplex* alpha, const scomplex* a, inc_t rs_a, inc_t cs_a,
+                const scomplex* b, inc_t rs_b, inc_t cs_b,
+                const scomplex* beta, scomplex* c, inc_t rs_c, inc_t cs_c );
+void bli_zgemm( trans_t transa, trans_t transb, dim_t m, dim_t n, dim_t k,
+                const dcomplex* alpha, const dcomplex* a, inc_t rs_a, inc_t cs_a,
+                const dcomplex* b, inc_t rs_b, inc_t cs_b,
+                const dcomplex* beta, dcomplex* c, inc_t rs_c, inc_t cs_c );
+void bli_cgemm_ex( trans_t transa, trans_t transb, dim_t m, dim_t n, dim_t k,
+                   const scomplex* alpha, const scomplex* a, inc_t rs_a, inc_t cs_a,
+                   const scomplex* b, inc_t rs_b, inc_t cs_b,
+                   const scomplex* beta, scomplex* c, inc_t rs_c, inc_t cs_c,
+                   const cntx_t* cntx, const rntm_t* rntm );
+void bli_zgemm_ex( trans_t transa, trans_t transb, dim_t m, dim_t n, dim_t k,
+                   const dcomplex* alpha, const dcomplex* a, inc_t rs_a, inc_t cs_a,
+                   const dcomplex* b, inc_t rs_b, inc_t cs_b,
+                   const dcomplex* beta, dcomplex* c, inc_t rs_c, inc_t cs_c,
+                   const cntx_t* cntx, const rntm_t* rntm );
+
+/* ---- structured level-3: typed API (frame/3/bli_l3_tapi_ex.h:60-225) --
+   gemmt: C := beta*C + alpha*op(A)*op(B) on the uploc triangle of the
+   m x m C; syrk: gemmt(A, A^T); syr2k: alpha*(op(A)op(B)^T + op(B)op(A)^T);
+   symm: C := beta*C + alpha*A*op(B) (left) or alpha*op(B)*A (right), A
+   symmetric in its uploa triangle (conja has no effect for real types);
+   trmm: B := alpha*op(A)*B (left) or alpha*B*op(A) (right), A triangular;
+   trmm3: C := beta*C + alpha*op(A)*op(B) (left) or alpha*op(B)*op(A). */
+#define BLIS_AMD_STRUC_PROTOS( ch, T ) \
+void bli_##ch##gemmt( uplo_t uploc, trans_t transa, trans_t transb, dim_t m, dim_t k, \
+                      const T* alpha, const T* a, inc_t rs_a, inc_t cs_a, \
+                      const T* b, inc_t rs_b, inc_t cs_b, \
+                      const T* beta, T* c, inc_t rs_c, inc_t cs_c ); \
+void bli_##ch##gemmt_ex( uplo_t uploc, trans_t transa, trans_t transb, dim_t m, dim_t k, \
+                         const T* alpha, const T* a, inc_t rs_a, inc_t cs_a, \
+                         const T* b, inc_t rs_b, inc_t cs_b, \
+                         const T* beta, T* c, inc_t rs_c, inc_t cs_c, \
+                         const cntx_t* cntx, const rntm_t* rntm ); \
+void bli_##ch##syrk( uplo_t uploc, trans_t transa, dim_t m, dim_t k, \
+                     const T* alpha, const T* a, inc_t rs_a, inc_t cs_a, \
+                     const T* beta, T* c, inc_t rs_c, inc_t cs_c ); \
+void bli_##ch##syrk_ex( uplo_t uploc, trans_t transa, dim_t m, dim_t k, \
+                        const T* alpha, const T* a, inc_t rs_a, inc_t cs_a, \
+                        const T* beta, T* c, inc_t rs_c, inc_t cs_c, \
+                        const cntx_t* cntx, const rntm_t* rntm ); \
+void bli_##ch##syr2k( uplo_t uploc, trans_t transa, trans_t transb, dim_t m, dim_t k, \
+                      const T* alpha, const T* a, inc_t rs_a, inc_t cs_a, \
+                      const T* b, inc_t rs_b, inc_t cs_b, \
+                      const T* beta, T* c, inc_t rs_c, inc_t cs_c ); \
+void bli_##ch##syr2k_ex( uplo_t uploc, trans_t transa, trans_t transb, dim_t m, dim_t k, \
+                         const T* alpha, const T* a, inc_t rs_a, inc_t cs_a, \
+                         const T* b, inc_t rs_b, inc_t cs_b, \
+                         const T* beta, T* c, inc_t rs_c, inc_t cs_c, \
+                         const cntx_t* cntx, const rntm_t* rntm ); \
+void bli_##ch##symm( side_t side, uplo_t uploa, conj_t conja, trans_t transb, \
+                     dim_t m, dim_t n, const T* alpha, const T* a, inc_t rs_a, inc_t cs_a, \
+                     const T* b, inc_t rs_b, inc_t cs_b, \
+                     const T* beta, T* c, inc_t rs_c, inc_t cs_c ); \
+void bli_##ch##symm_ex( side_t side, uplo_t uploa, conj_t conja, trans_t transb, \
+                        dim_t m, dim_t n, const T* alpha, const T* a, inc_t rs_a, inc_t cs_a, \
+                        const T* b, inc_t rs_b, inc_t cs_b, \
+                        const T* beta, T* c, inc_t rs_c, inc_t cs_c, \
+                        const cntx_t* cntx, const rntm_t* rntm ); \
+void bli_##ch##trmm( side_t side, uplo_t uploa, trans_t transa, diag_t diaga, \
+                     dim_t m, dim_t n, const T* alpha, const T* a, inc_t rs_a, inc_t cs_a, \
+                     T* b, inc_t rs_b, inc_t cs_b ); \
+void bli_##ch##trmm_ex( side_t side, uplo_t uploa, trans_t transa, diag_t diaga, \
+                        dim_t m, dim_t n, const T* alpha, const T* a, inc_t rs_a, inc_t cs_a, \
+                        T* b, inc_t rs_b, inc_t cs_b, \
+                        const cntx_t* cntx, const rntm_t* rntm ); \
+void bli_##ch##trmm3( side_t side, uplo_t uploa, trans_t transa, diag_t diaga, \
+                      trans_t transb, dim_t m, dim_t n, \
+                      const T* alpha, const T* a, inc_t rs_a, inc_t cs_a, \
+                      const T* b, inc_t rs_b, inc_t cs_b, \
+                      const T* beta, T* c, inc_t rs_c, inc_t cs_c ); \
+void bli_##ch##trmm3_ex( side_t side, uplo_t uploa, trans_t transa, diag_t diaga, \
+                         trans_t transb, dim_t m, dim_t n, \
+                         const T* alpha, const T* a, inc_t rs_a, inc_t cs_a, \
+                         const T* b, inc_t rs_b, inc_t cs_b, \
+                         const T* beta, T* c, inc_t rs_c, inc_t cs_c, \
+                         const cntx_t* cntx, const rntm_t* rntm );
+BLIS_AMD_STRUC_PROTOS( s, float )
+BLIS_AMD_STRUC_PROTOS( d, double )
+
 /* Typed batched GEMM. No counterpart exists in the reference, whose batch
    interface is only the BLAS-style ?gemm_batch_ (frame/compat/extra/
    bla_gemm_batch.c:135-246). Same group semantics: group g holds
@@ -453,6 +552,36 @@ void bli_gemm_ex( const obj_t* alpha, const obj_t* a, const obj_t* b,
 void bli_trsm( side_t side, const obj_t* alpha, const obj_t* a, const obj_t* b );
 void bli_trsm_ex( side_t side, const obj_t* alpha, const obj_t* a, const obj_t* b,
                   const cntx_t* cntx, const rntm_t* rntm );
+
+/* Structured level-3 object API (frame/3/bli_l3_oapi_ex.c:151-690). The
+   stored triangle of gemmt's C, and the structure (uplo, diag, diagonal
+   offset) of symm's and trmm's A, come from the objects' info fields. */
+void bli_gemmt( const obj_t* alpha, const obj_t* a, const obj_t* b,
+                const obj_t* beta, const obj_t* c );
+void bli_gemmt_ex( const obj_t* alpha, const obj_t* a, const obj_t* b,
+                   const obj_t* beta, const obj_t* c,
+                   const cntx_t* cntx, const rntm_t* rntm );
+void bli_syrk( const obj_t* alpha, const obj_t* a, const obj_t* beta, const obj_t* c );
+void bli_syrk_ex( const obj_t* alpha, const obj_t* a, const obj_t* beta, const obj_t* c,
+                  const cntx_t* cntx, const rntm_t* rntm );
+void bli_syr2k( const obj_t* alpha, const obj_t* a, const obj_t* b,
+                const obj_t* beta, const obj_t* c );
+void bli_syr2k_ex( const obj_t* alpha, const obj_t* a, const obj_t* b,
+                   const obj_t* beta, const obj_t* c,
+                   const cntx_t* cntx, const rntm_t* rntm );
+void bli_symm( side_t side, const obj_t* alpha, const obj_t* a, const obj_t* b,
+               const obj_t* beta, const obj_t* c );
+void bli_symm_ex( side_t side, const obj_t* alpha, const obj_t* a, const obj_t* b,
+                  const obj_t* beta, const obj_t* c,
+                  const cntx_t* cntx, const rntm_t* rntm );
+void bli_trmm( side_t side, const obj_t* alpha, const obj_t* a, const obj_t* b );
+void bli_trmm_ex( side_t side, const obj_t* alpha, const obj_t* a, const obj_t* b,
+                  const cntx_t* cntx, const rntm_t* rntm );
+void bli_trmm3( side_t side, const obj_t* alpha, const obj_t* a, const obj_t* b,
+                const obj_t* beta, const obj_t* c );
+void bli_trmm3_ex( side_t side, const obj_t* alpha, const obj_t* a, const obj_t* b,
+                   const obj_t* beta, const obj_t* c,
+                   const cntx_t* cntx, const rntm_t* rntm );
 
 /* ---- level-1v / level-2 typed API ------------------------------------ */
 
@@ -548,6 +677,32 @@ void dgemm_batch_( const f77_char* transa_array, const f77_char* transb_array,
                    const double* beta_array,
                    double** c_array, const f77_int* ldc_array,
                    const f77_int* group_count, const f77_int* group_size );
+void cgemm_( const f77_char* transa, const f77_char* transb,
+             const f77_int* m, const f77_int* n, const f77_int* k,
+             const scomplex* alpha, const scomplex* a, const f77_int* lda,
+             const scomplex* b, const f77_int* ldb,
+             const scomplex* beta, scomplex* c, const f77_int* ldc );
+void zgemm_( const f77_char* transa, const f77_char* transb,
+             const f77_int* m, const f77_int* n, const f77_int* k,
+             const dcomplex* alpha, const dcomplex* a, const f77_int* lda,
+             const dcomplex* b, const f77_int* ldb,
+             const dcomplex* beta, dcomplex* c, const f77_int* ldc );
+void cgemm_batch_( const f77_char* transa_array, const f77_char* transb_array,
+                   const f77_int* m_array, const f77_int* n_array, const f77_int* k_array,
+                   const scomplex* alpha_array,
+                   const scomplex** a_array, const f77_int* lda_array,
+                   const scomplex** b_array, const f77_int* ldb_array,
+                   const scomplex* beta_array,
+                   scomplex** c_array, const f77_int* ldc_array,
+                   const f77_int* group_count, const f77_int* group_size );
+void zgemm_batch_( const f77_char* transa_array, const f77_char* transb_array,
+                   const f77_int* m_array, const f77_int* n_array, const f77_int* k_array,
+                   const dcomplex* alpha_array,
+                   const dcomplex** a_array, const f77_int* lda_array,
+                   const dcomplex** b_array, const f77_int* ldb_array,
+                   const dcomplex* beta_array,
+                   dcomplex** c_array, const f77_int* ldc_array,
+                   const f77_int* group_count, const f77_int* group_size );
 void sgemv_( const f77_char* transa, const f77_int* m, const f77_int* n,
              const float* alpha, const float* a, const f77_int* lda,
              const float* x, const f77_int* incx,
@@ -564,6 +719,31 @@ float  sdot_( const f77_int* n, const float* x, const f77_int* incx,
               const float* y, const f77_int* incy );
 double ddot_( const f77_int* n, const double* x, const f77_int* incx,
               const double* y, const f77_int* incy );
+
+/* Structured level-3 BLAS (frame/compat/bla_symm.c, bla_trmm.c, bla_syrk.c,
+   bla_syr2k.c, extra/bla_gemmt.c). */
+#define BLIS_AMD_STRUC_BLAS( ch, T ) \
+void ch##symm_( const f77_char* side, const f77_char* uploa, \
+                const f77_int* m, const f77_int* n, const T* alpha, \
+                const T* a, const f77_int* lda, const T* b, const f77_int* ldb, \
+                const T* beta, T* c, const f77_int* ldc ); \
+void ch##trmm_( const f77_char* side, const f77_char* uploa, \
+                const f77_char* transa, const f77_char* diaga, \
+                const f77_int* m, const f77_int* n, const T* alpha, \
+                const T* a, const f77_int* lda, T* b, const f77_int* ldb ); \
+void ch##syrk_( const f77_char* uploc, const f77_char* transa, \
+                const f77_int* m, const f77_int* k, const T* alpha, \
+                const T* a, const f77_int* lda, const T* beta, T* c, const f77_int* ldc ); \
+void ch##syr2k_( const f77_char* uploc, const f77_char* trans, \
+                 const f77_int* m, const f77_int* k, const T* alpha, \
+                 const T* a, const f77_int* lda, const T* b, const f77_int* ldb, \
+                 const T* beta, T* c, const f77_int* ldc ); \
+void ch##gemmt_( const f77_char* uploc, const f77_char* transa, const f77_char* transb, \
+                 const f77_int* m, const f77_int* k, const T* alpha, \
+                 const T* a, const f77_int* lda, const T* b, const f77_int* ldb, \
+                 const T* beta, T* c, const f77_int* ldc );
+BLIS_AMD_STRUC_BLAS( s, float )
+BLIS_AMD_STRUC_BLAS( d, double )
 
 /* xerbla_ prints the netlib message and returns (frame/compat/f2c/
    bla_xerbla.c). It is a weak symbol so that a program may supply its own,

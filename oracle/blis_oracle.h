@@ -86,6 +86,46 @@ void oracle_dgemm_dd_rows(int transa, int transb, int64_t m, int64_t n, int64_t 
                           double beta, double* c, int64_t rs_c, int64_t cs_c,
                           int64_t r0, int64_t r1, int nthreads);
 
+/* Structured level-3 (blis_oracle_struc.c). uplo/side/diag/trans use the
+   encodings above; C of gemmt/syrk/syr2k is m x m. */
+#define ORACLE_STRUC_DECL(T, ch)                                                       \
+void oracle_##ch##gemmt(int uploc, int transa, int transb, int64_t m, int64_t k,       \
+                        T alpha, const T* a, int64_t rs_a, int64_t cs_a,               \
+                        const T* b, int64_t rs_b, int64_t cs_b, T beta,                \
+                        T* c, int64_t rs_c, int64_t cs_c);                             \
+void oracle_##ch##syrk(int uploc, int transa, int64_t m, int64_t k, T alpha,           \
+                       const T* a, int64_t rs_a, int64_t cs_a, T beta,                 \
+                       T* c, int64_t rs_c, int64_t cs_c);                              \
+void oracle_##ch##syr2k(int uploc, int transa, int transb, int64_t m, int64_t k,       \
+                        T alpha, const T* a, int64_t rs_a, int64_t cs_a,               \
+                        const T* b, int64_t rs_b, int64_t cs_b, T beta,                \
+                        T* c, int64_t rs_c, int64_t cs_c);                             \
+void oracle_##ch##symm(int side, int uploa, int transb, int64_t m, int64_t n,         \
+                       T alpha, const T* a, int64_t rs_a, int64_t cs_a,                \
+                       const T* b, int64_t rs_b, int64_t cs_b, T beta,                 \
+                       T* c, int64_t rs_c, int64_t cs_c);                              \
+void oracle_##ch##trmm3(int side, int uploa, int transa, int diaga, int transb,       \
+                        int64_t m, int64_t n, T alpha, const T* a, int64_t rs_a,       \
+                        int64_t cs_a, const T* b, int64_t rs_b, int64_t cs_b, T beta,  \
+                        T* c, int64_t rs_c, int64_t cs_c);                             \
+void oracle_##ch##trmm(int side, int uploa, int transa, int diaga, int64_t m,         \
+                       int64_t n, T alpha, const T* a, int64_t rs_a, int64_t cs_a,     \
+                       T* b, int64_t rs_b, int64_t cs_b);
+ORACLE_STRUC_DECL(double, d)
+ORACLE_STRUC_DECL(float, s)
+
+/* Complex GEMM by the 1m method (blis_oracle_cx.c). Matrices are arrays of
+   interleaved (re, im) pairs; strides count complex elements; transa/transb
+   may carry the conjugate bit 0x10 besides the transpose bit 0x8. */
+void oracle_zgemm(int transa, int transb, int64_t m, int64_t n, int64_t k,
+                  double alpha_re, double alpha_im, const double* a, int64_t rs_a,
+                  int64_t cs_a, const double* b, int64_t rs_b, int64_t cs_b,
+                  double beta_re, double beta_im, double* c, int64_t rs_c, int64_t cs_c);
+void oracle_cgemm(int transa, int transb, int64_t m, int64_t n, int64_t k,
+                  float alpha_re, float alpha_im, const float* a, int64_t rs_a,
+                  int64_t cs_a, const float* b, int64_t rs_b, int64_t cs_b,
+                  float beta_re, float beta_im, float* c, int64_t rs_c, int64_t cs_c);
+
 #ifdef __cplusplus
 }
 #endif

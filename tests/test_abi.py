@@ -19,19 +19,24 @@ import pytest
 from tests.conftest import ROOT
 
 LIB = ROOT / "blis_amd" / "lib" / "libblis_amd.so"
-HEADERS = [ROOT / "include" / "blis.h", ROOT / "include" / "blis_amd.h"]
+HEADERS = [ROOT / "include" / "blis.h", ROOT / "include" / "blis_amd.h",
+           ROOT / "include" / "cblas.h"]
 
 
 def declared_functions():
+    """Function names declared by the headers, after the C preprocessor (some
+    families are declared through macros, one expansion per datatype)."""
     names = set()
     for h in HEADERS:
-        text = h.read_text()
+        text = subprocess.run(["gcc", "-E", "-P", "-DBLIS_IS_BUILDING_LIBRARY",
+                               f"-I{ROOT / 'include'}", "-x", "c", str(h)],
+                              capture_output=True, text=True, check=True).stdout
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"static inline[^{]*\{[^}]*\}", "", text)
         for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(\s*(?:const\s+)?[A-Za-z_]", text):
             name = m.group(1)
-            if name.startswith(("bli_", "bli_amd_")) or name.endswith("_"):
-                if name in ("defined",):
+            if name.startswith(("bli_", "bli_amd_", "cblas_")) or name.endswith("_"):
+                if name in ("defined", "__aligned__"):
                     continue
                 names.add(name)
     return names
@@ -131,3 +136,42 @@ def test_python_obj_t_mirror_matches():
     assert blis_amd.obj_t.info.offset == 48
     assert blis_amd.obj_t.buffer.offset == 64
     assert blis_amd.obj_t.scalar.offset == 96
+
+
+def test_cblas_declared_and_exported():
+    decl = declared_functions()
+    want = {"cblas_dgemm", "cblas_sgemm", "cblas_zgemm", "cblas_cgemm", "cblas_dgemm_batch",
+            "cblas_zgemm_batch", "cblas_dtrsm", "cblas_dtrmm", "cblas_dsymm", "cblas_dsyrk",
+            "cblas_dsyr2k", "cblas_dgemmt", "cblas_dgemv", "cblas_ddot", "cblas_daxpy",
+            "cblas_xerbla"}
+    assert want <= decl
+    assert want <= exported()
+
+
+@pytest.mark.parametrize("call,param,routine", [
+    ("cblas_dgemm(99, CblasNoTrans, CblasNoTrans, 2, 2, 2, 1.0, a, 2, a, 2, 0.0, a, 2)",
+     1, "cblas_dgemm"),
+    ("cblas_dgemm(CblasColMajor, 7, CblasNoTrans, 2, 2, 2, 1.0, a, 2, a, 2, 0.0, a, 2)",
+     2, "cblas_dgemm"),
+    ("cblas_dgemm(CblasColMajor, CblasNoTrans, 7, 2, 2, 2, 1.0, a, 2, a, 2, 0.0, a, 2)",
+     3, "cblas_dgemm"),
+    ("cblas_dtrsm(CblasRowMajor, CblasLeft, 7, CblasNoTrans, CblasUnit, 2, 2, 1.0, a, 2, a, 2)",
+     3, "cblas_dtrsm"),
+    ("cblas_dsyrk(CblasRowMajor, 9, CblasNoTrans, 2, 2, 1.0, a, 2, 0.0, a, 2)",
+     3, "cblas_dsyrk"),
+])
+def test_cblas_argument_errors(tmp_path, call, param, routine):
+    """An illegal enum argument is reported by cblas_xerbla with the
+    reference's parameter number and ends the process with exit(-1)
+    (frame/compat/cblas/src/cblas_xerbla.c). No device is touched: the
+    check comes before any computation."""
+    src = tmp_path / "probe.c"
+    src.write_text(f"""#include "cblas.h"
+int main(void) {{ double a[4] = {{0}}; {call}; return 0; }}
+""")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-o", str(exe), str(src), f"-I{ROOT / 'include'}",
+                    f"-L{LIB.parent}", "-lblis_amd", f"-Wl,-rpath,{LIB.parent}"], check=True)
+    p = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 255, (p.returncode, p.stderr)
+    assert f"Parameter {param} to routine {routine} was incorrect" in p.stderr, p.stderr

@@ -1,6 +1,7 @@
 """Pin the CPU oracle before trusting it (CPU only).
 
-1. The reference's own known-answer drivers: netlib ?blat3 (GEMM, TRSM) and
+1. The reference's own known-answer drivers: netlib ?blat3 (GEMM, SYMM, TRMM,
+   TRSM, SYRK, SYR2K) and
    ?blat2 (GEMV), compiled from the reference tree by oracle/Makefile and
    linked against the oracle's BLAS symbols. They check every trans/side/
    uplo/diag combination, alpha/beta in {0, 1, 0.7}/{0, 1, 1.3}, edge sizes
@@ -31,7 +32,7 @@ NETLIB_IN = ROOT / "tests" / "golden" / "netlib"
 
 
 def _ensure_ref_built():
-    drivers = ("dblat3", "dblat2", "sblat3", "sblat2", "dblat1", "sblat1")
+    drivers = ("dblat3", "dblat2", "sblat3", "sblat2", "zblat3", "cblat3", "dblat1", "sblat1")
     if not all((REF_DIR / f"{d}_oracle.x").exists() for d in drivers):
         if not Path("/root/reference").exists():
             pytest.skip("netlib drivers not built and the reference tree is absent")
@@ -53,8 +54,14 @@ def run_netlib(driver: str, suffix: str) -> str:
 
 
 NETLIB_CASES = [
-    ("dblat3", ["DGEMM", "DTRSM"], {"DGEMM": 17496, "DTRSM": 2592}),
-    ("sblat3", ["SGEMM", "STRSM"], {"SGEMM": 17496, "STRSM": 2592}),
+    ("dblat3", ["DGEMM", "DSYMM", "DTRMM", "DTRSM", "DSYRK", "DSYR2K"],
+     {"DGEMM": 17496, "DSYMM": 1296, "DTRMM": 2592, "DTRSM": 2592, "DSYRK": 1944,
+      "DSYR2K": 1944}),
+    ("sblat3", ["SGEMM", "SSYMM", "STRMM", "STRSM", "SSYRK", "SSYR2K"],
+     {"SGEMM": 17496, "SSYMM": 1296, "STRMM": 2592, "STRSM": 2592, "SSYRK": 1944,
+      "SSYR2K": 1944}),
+    ("zblat3", ["ZGEMM"], {"ZGEMM": 17496}),
+    ("cblat3", ["CGEMM"], {"CGEMM": 17496}),
     ("dblat2", ["DGEMV"], {"DGEMV": 3460}),
     ("sblat2", ["SGEMV"], {"SGEMV": 3460}),
 ]
@@ -62,9 +69,9 @@ NETLIB_CASES = [
 
 def check_netlib_output(text: str, routines, calls):
     for r in routines:
-        assert f"{r}  PASSED THE TESTS OF ERROR-EXITS" in text, text[-3000:]
-        assert f"{r}  PASSED THE COMPUTATIONAL TESTS ( {calls[r]:5d} CALLS)" in text or \
-            f"{r}  PASSED THE COMPUTATIONAL TESTS ({calls[r]:6d} CALLS)" in text, text[-3000:]
+        assert f"{r:<6} PASSED THE TESTS OF ERROR-EXITS" in text, text[-3000:]
+        assert f"{r:<6} PASSED THE COMPUTATIONAL TESTS ( {calls[r]:5d} CALLS)" in text or \
+            f"{r:<6} PASSED THE COMPUTATIONAL TESTS ({calls[r]:6d} CALLS)" in text, text[-3000:]
     for bad in ("FAILED", "SUSPECT", "******* FATAL ERROR"):
         assert bad not in text, text[-3000:]
 
@@ -408,3 +415,175 @@ def test_oracle_gemm_batch_is_a_loop_of_gemm(oracle, ch):
             oracle.gemm(gta, gtb, al, a, b, be, ref)
             np.testing.assert_array_equal(outs[idx], ref)
             idx += 1
+
+
+# ---- structured level-3: extended-precision pins ---------------------------
+
+def _tri(a, lower, unit=False):
+    t = np.tril(a) if lower else np.triu(a)
+    if unit:
+        np.fill_diagonal(t, 1.0)
+    return t
+
+
+def _sym(a, lower):
+    t = np.tril(a) if lower else np.triu(a)
+    return t + (np.tril(t, -1).T if lower else np.triu(t, 1).T)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_struc_oracle_against_extended_precision(oracle, dtype):
+    """gemmt/syrk/syr2k/symm/trmm/trmm3 of the oracle against products formed
+    in long double from explicitly structured dense matrices; the unstored
+    triangle of C and of A (NaN-filled) must stay untouched / unread."""
+    rng = np.random.default_rng(11)
+    u = unit_roundoff(dtype)
+    L, U = oracle.LOWER, oracle.UPPER
+    ld = np.longdouble
+
+    def close(got, ref, scale, k):
+        err = np.max(np.abs(got.astype(ld) - ref)) / scale
+        return err <= 4 * (k + 2) * u
+
+    for m, k in [(1, 1), (7, 5), (33, 17)]:
+        a = np.asfortranarray(rng.uniform(-1, 1, (m, k)).astype(dtype))
+        b = np.asfortranarray(rng.uniform(-1, 1, (m, k)).astype(dtype))
+        for uplo in (L, U):
+            low = uplo == L
+            mask = np.tril(np.ones((m, m), bool)) if low else np.triu(np.ones((m, m), bool))
+            c0 = np.asfortranarray(rng.uniform(-1, 1, (m, m)).astype(dtype))
+            c0[~mask] = np.nan
+            # gemmt with B given transposed (A * B^T)
+            c = c0.copy(order="F")
+            oracle.gemmt(uplo, "n", "t", 0.5, a, b, 1.5, c)
+            ref = 0.5 * (a.astype(ld) @ b.astype(ld).T) + 1.5 * c0.astype(ld)
+            assert np.isnan(c[~mask]).all()
+            assert close(c[mask], ref[mask], 2.0 * k + 2, k)
+            # syrk and syr2k
+            c = c0.copy(order="F")
+            oracle.syrk(uplo, "n", 0.5, a, 1.5, c)
+            ref = 0.5 * (a.astype(ld) @ a.astype(ld).T) + 1.5 * c0.astype(ld)
+            assert close(c[mask], ref[mask], 2.0 * k + 2, k) and np.isnan(c[~mask]).all()
+            c = c0.copy(order="F")
+            oracle.syr2k(uplo, "n", "n", 0.5, a, b, 1.5, c)
+            ref = 0.5 * (a.astype(ld) @ b.astype(ld).T + b.astype(ld) @ a.astype(ld).T) \
+                + 1.5 * c0.astype(ld)
+            assert close(c[mask], ref[mask], 4.0 * k + 2, 2 * k) and np.isnan(c[~mask]).all()
+            # symm / trmm / trmm3 with the unstored triangle of A set to NaN
+            sq = np.asfortranarray(rng.uniform(-1, 1, (m, m)).astype(dtype))
+            sq_nan = sq.copy(order="F")
+            sq_nan[~mask] = np.nan
+            bb = np.asfortranarray(rng.uniform(-1, 1, (m, k)).astype(dtype))
+            cc0 = np.asfortranarray(rng.uniform(-1, 1, (m, k)).astype(dtype))
+            cc = cc0.copy(order="F")
+            oracle.symm(oracle.LEFT, uplo, "n", 0.5, sq_nan, bb, 1.5, cc)
+            ref = 0.5 * (_sym(sq.astype(ld), low) @ bb.astype(ld)) + 1.5 * cc0.astype(ld)
+            assert close(cc, ref, 2.0 * m + 2, m)
+            for unit in (False, True):
+                tri_nan = sq_nan.copy(order="F")
+                if unit:
+                    np.fill_diagonal(tri_nan, np.nan)
+                bt = bb.copy(order="F")
+                oracle.trmm(oracle.LEFT, uplo, "t", oracle.UNIT if unit else oracle.NONUNIT,
+                            0.5, tri_nan, bt)
+                ref = 0.5 * (_tri(sq.astype(ld), low, unit).T @ bb.astype(ld))
+                assert close(bt, ref, 2.0 * m, m)
+                cr0 = np.asfortranarray(rng.uniform(-1, 1, (k, m)).astype(dtype))
+                cr = cr0.copy(order="F")
+                oracle.trmm3(oracle.RIGHT, uplo, "n", oracle.UNIT if unit else oracle.NONUNIT,
+                             "t", 0.5, tri_nan, bb, 1.5, cr)
+                ref = 0.5 * (bb.astype(ld).T @ _tri(sq.astype(ld), low, unit)) + \
+                    1.5 * cr0.astype(ld)
+                assert close(cr, ref, 2.0 * m + 2, m)
+
+
+def test_struc_checks_reject_planted_mistakes(oracle):
+    """The extended-precision check above rejects a trmm that uses the wrong
+    triangle and a symm that mirrors the stored triangle the wrong way."""
+    rng = np.random.default_rng(5)
+    m, n = 9, 4
+    sq = np.asfortranarray(rng.uniform(-1, 1, (m, m)))
+    bb = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    bt = bb.copy(order="F")
+    oracle.trmm(oracle.LEFT, oracle.LOWER, "n", oracle.NONUNIT, 1.0, sq, bt)
+    wrong_tri = np.triu(sq) @ bb
+    assert np.max(np.abs(bt - wrong_tri)) > 1e-3
+    cc = np.zeros((m, n), order="F")
+    oracle.symm(oracle.LEFT, oracle.LOWER, "n", 1.0, sq, bb, 0.0, cc)
+    wrong_sym = (np.tril(sq) + np.tril(sq, -1)) @ bb   # mirror added without transposing
+    assert np.max(np.abs(cc - wrong_sym)) > 1e-3
+
+
+
+# ---- complex (1m) and mixed-datatype GEMM: extended-precision pins --------
+
+def _op(x, t):
+    return {"n": x, "t": x.T, "r": np.conj(x), "c": np.conj(x).T}[t]
+
+
+@pytest.mark.parametrize("cdt", [np.complex128, np.complex64])
+def test_zgemm_oracle_against_extended_precision(oracle, cdt):
+    """The 1m oracle against the product formed in complex long double, for
+    every transpose/conjugate combination, a complex alpha and beta, beta = 0
+    over a NaN C, and a C with general strides."""
+    rng = np.random.default_rng(3)
+    u = unit_roundoff(np.float64 if cdt == np.complex128 else np.float32)
+    ld = np.clongdouble
+
+    def rnd(r, c):
+        return (rng.uniform(-1, 1, (r, c)) + 1j * rng.uniform(-1, 1, (r, c))).astype(cdt)
+
+    for (m, n, k) in [(1, 1, 1), (7, 5, 3), (19, 23, 31)]:
+        for ta in "ntrc":
+            for tb in "ntrc":
+                a = np.asfortranarray(rnd(*((k, m) if ta in "tc" else (m, k))))
+                b = np.asfortranarray(rnd(*((n, k) if tb in "tc" else (k, n))))
+                c0 = np.asfortranarray(rnd(m, n))
+                al, be = 0.7 - 0.9j, 1.3 - 1.1j
+                c = c0.copy(order="F")
+                oracle.zgemm(ta, tb, al, a, b, be, c)
+                ref = al * (_op(a.astype(ld), ta) @ _op(b.astype(ld), tb)) + be * c0.astype(ld)
+                err = np.max(np.abs(c.astype(ld) - ref))
+                assert err <= 8 * (k + 2) * u * (2 * k + 2), (m, n, k, ta, tb, err)
+    # beta = 0 ignores a NaN C; general strides
+    a, b = np.asfortranarray(rnd(6, 4)), np.asfortranarray(rnd(4, 5))
+    big = np.full((12, 15), np.nan, dtype=cdt)
+    cv = big[::2, ::3]
+    oracle.zgemm("n", "n", 1.0, a, b, 0.0, cv)
+    assert np.allclose(cv, a @ b, atol=50 * u)
+    assert np.isnan(big[1::2]).all()
+
+
+def test_zgemm_check_rejects_planted_mistake(oracle):
+    """The check above rejects a product with the conjugation dropped."""
+    rng = np.random.default_rng(4)
+    a = np.asfortranarray(rng.uniform(-1, 1, (5, 4)) + 1j * rng.uniform(-1, 1, (5, 4)))
+    b = np.asfortranarray(rng.uniform(-1, 1, (4, 3)) + 1j * rng.uniform(-1, 1, (4, 3)))
+    c = np.zeros((5, 3), dtype=complex, order="F")
+    oracle.zgemm("r", "n", 1.0, a, b, 0.0, c)
+    assert np.max(np.abs(c - a @ b)) > 1e-3
+
+
+def test_gemm_md_oracle_rules(oracle):
+    """The mixed-datatype oracle against the documented rules, formed
+    directly in long double: real C += complex A * complex B keeps the real
+    part of the complex product; complex C += real A * real B updates only
+    the real parts; float C from double A, B in double precision."""
+    rng = np.random.default_rng(9)
+    ld, cld = np.longdouble, np.clongdouble
+    m, n, k = 9, 7, 5
+    az = rng.uniform(-1, 1, (m, k)) + 1j * rng.uniform(-1, 1, (m, k))
+    bz = rng.uniform(-1, 1, (k, n)) + 1j * rng.uniform(-1, 1, (k, n))
+    cr = rng.uniform(-1, 1, (m, n))
+    got = oracle.gemm_md("n", "n", 0.5, az, bz, 2.0, cr, True)
+    ref = 0.5 * (az.astype(cld) @ bz.astype(cld)).real + 2.0 * cr.astype(ld)
+    assert np.max(np.abs(got - ref)) < 1e-13
+    ar, br = rng.uniform(-1, 1, (m, k)), rng.uniform(-1, 1, (k, n))
+    cz = rng.uniform(-1, 1, (m, n)) + 1j * rng.uniform(-1, 1, (m, n))
+    got = oracle.gemm_md("n", "n", 0.5, ar, br, 2.0, cz, True)
+    assert np.max(np.abs(got.real - (0.5 * ar.astype(ld) @ br.astype(ld) + 2 * cz.real))) < 1e-13
+    assert np.array_equal(got.imag, cz.imag)
+    cf = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    got = oracle.gemm_md("t", "n", 1.0, ar.T.copy(), br, 1.0, cf, True)
+    ref = (ar.astype(ld) @ br.astype(ld) + cf.astype(ld)).astype(np.float32)
+    assert got.dtype == np.float32 and np.array_equal(got, ref)
