@@ -125,6 +125,9 @@ def _sig(lib):
         s(f"{ch}gemmt_", None, PC, PC, PC, PF, PF, PT, P, PF, P, PF, PT, P, PF)
         s(f"bli_amd_{ch}gemm_split", C.c_int, C.c_int, C.c_int, dim_t, dim_t, dim_t, PT, P,
           inc_t, inc_t, P, inc_t, inc_t, PT, P, inc_t, inc_t, C.c_int, C.POINTER(C.c_double))
+    for ch in ("z", "c"):
+        s(f"bli_{ch}gemm", None, C.c_int, C.c_int, dim_t, dim_t, dim_t, P, P, inc_t, inc_t, P,
+          inc_t, inc_t, P, P, inc_t, inc_t)
     s("bli_amd_gemm_batch_plan_execute", None, P, C.c_int)
     s("bli_amd_gemm_batch_plan_destroy", None, P)
     s("bli_amd_gemm_batch_plan_work_items", C.c_int, P)

@@ -54,6 +54,7 @@ SOURCES = [
     "complex.hip",
     "api_complex.cpp",
     "cblas.cpp",
+    "cntx.cpp",
 ]
 
 

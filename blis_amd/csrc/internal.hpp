@@ -22,6 +22,10 @@ namespace blis_amd {
 
 enum class Dt : int { F32 = 0, F64 = 1 };
 
+// Largest m, n and k of the single-call small-problem path (BLIS's
+// MT/NT/KT thresholds, frame/base/bli_cntx.h:186-193).
+constexpr int64_t SMALL_GEMM_MAX_DIM = 256;
+
 inline size_t dt_size(Dt d) { return d == Dt::F64 ? 8 : 4; }
 
 // Strided 2-D view of device memory: element (i, j) lives at

@@ -18,6 +18,7 @@
  *   level-2 typed         frame/2/bli_l2_tapi.c:43-123 (gemv)
  *   structured level-3    frame/3/bli_l3_tapi_ex.h:60-225, bli_l3_oapi_ex.c:151-690
  *   complex / mixed GEMM  frame/3/bli_l3_ind.c (1m), docs/MixedDatatypes.md
+ *   registry / info       frame/base/bli_gks.c, bli_cntx.h:73-204, bli_info.c
  *   BLAS compatibility    frame/compat/bla_gemm.c:122-260, bla_trsm.c:121-216,
  *                         bla_gemv.c:41-141, bla_axpy.c:41-85, bla_dot.c:41-88,
  *                         extra/bla_gemm_batch.c:135-246
@@ -306,10 +307,103 @@ typedef struct obj_s {
     dim_t         n_panel;
 } obj_t;
 
-/* Context and runtime objects are accepted for signature compatibility and
-   ignored: blocking and parallelism are chosen by the GPU launch planner. */
-typedef struct cntx_s cntx_t;
+/* ---- kernel/blocksize registry types (bli_type_defs.h:583-935, 1131-1150) --
+   Same enumerator values as BLIS. The context itself is this library's own
+   structure: its fields are read through the bli_cntx_* functions below. */
+
+#define BLIS_NUM_FP_TYPES 4
+
+#define BLIS_NTYPE_KER_SHIFT 28
+#define BLIS_NTYPE_KER_BITS  ( 0xFu << BLIS_NTYPE_KER_SHIFT )
+#define BLIS_1TYPE_KER       ( 0u << BLIS_NTYPE_KER_SHIFT )
+#define BLIS_2TYPE_KER       ( 1u << BLIS_NTYPE_KER_SHIFT )
+#define bli_ker_idx( ker )   ( ( kerid_t )( ( ker ) & ~BLIS_NTYPE_KER_BITS ) )
+#define bli_ker_ntype( ker ) ( ( ( ( ker ) & BLIS_NTYPE_KER_BITS ) >> BLIS_NTYPE_KER_SHIFT ) + 1 )
+#define BLIS_VA_END ( ( kerid_t )-1 )
+
+typedef uint32_t kerid_t;
+
+typedef enum ind_e { BLIS_1M = 0, BLIS_NAT, BLIS_NUM_IND_METHODS } ind_t;
+
+typedef enum ukr_e {
+    BLIS_ADDV_KER = BLIS_1TYPE_KER, BLIS_AMAXV_KER, BLIS_AXPBYV_KER, BLIS_AXPYV_KER,
+    BLIS_COPYV_KER, BLIS_DOTV_KER, BLIS_DOTXV_KER, BLIS_INVERTV_KER, BLIS_INVSCALV_KER,
+    BLIS_SCALV_KER, BLIS_SCAL2V_KER, BLIS_SETV_KER, BLIS_SUBV_KER, BLIS_SWAPV_KER,
+    BLIS_XPBYV_KER, BLIS_AXPY2V_KER, BLIS_DOTAXPYV_KER,
+    BLIS_AXPYF_KER, BLIS_DOTXF_KER, BLIS_DOTXAXPYF_KER,
+    BLIS_GEMMTRSM_L_UKR, BLIS_GEMMTRSM_U_UKR, BLIS_TRSM_L_UKR, BLIS_TRSM_U_UKR,
+    BLIS_GEMMTRSM1M_L_UKR, BLIS_GEMMTRSM1M_U_UKR,
+    BLIS_GEMMSUP_RRR_UKR, BLIS_GEMMSUP_RRC_UKR, BLIS_GEMMSUP_RCR_UKR, BLIS_GEMMSUP_RCC_UKR,
+    BLIS_GEMMSUP_CRR_UKR, BLIS_GEMMSUP_CRC_UKR, BLIS_GEMMSUP_CCR_UKR, BLIS_GEMMSUP_CCC_UKR,
+    BLIS_GEMMSUP_XXX_UKR,
+    BLIS_NUM_UKRS_, BLIS_NUM_UKRS = bli_ker_idx( BLIS_NUM_UKRS_ ),
+    BLIS_PACKM_KER = BLIS_2TYPE_KER, BLIS_PACKM_1ER_KER, BLIS_PACKM_RO_KER,
+    BLIS_PACKM_DIAG_KER, BLIS_PACKM_DIAG_1ER_KER, BLIS_PACKM_DIAG_RO_KER,
+    BLIS_UNPACKM_KER,
+    BLIS_GEMM_UKR,
+    BLIS_GEMM1M_UKR,
+    BLIS_GEMM_CCR_UKR, BLIS_GEMM_RCC_UKR, BLIS_GEMM_CRR_UKR,
+    BLIS_NUM_UKR2S_, BLIS_NUM_UKR2S = bli_ker_idx( BLIS_NUM_UKR2S_ ),
+    BLIS_UKRS_END_ = BLIS_VA_END
+} ukr_t;
+
+typedef enum ukr_pref_e {
+    BLIS_GEMM_UKR_ROW_PREF, BLIS_GEMMTRSM_L_UKR_ROW_PREF, BLIS_GEMMTRSM_U_UKR_ROW_PREF,
+    BLIS_TRSM_L_UKR_ROW_PREF, BLIS_TRSM_U_UKR_ROW_PREF,
+    BLIS_GEMMSUP_RRR_UKR_ROW_PREF, BLIS_GEMMSUP_RRC_UKR_ROW_PREF,
+    BLIS_GEMMSUP_RCR_UKR_ROW_PREF, BLIS_GEMMSUP_RCC_UKR_ROW_PREF,
+    BLIS_GEMMSUP_CRR_UKR_ROW_PREF, BLIS_GEMMSUP_CRC_UKR_ROW_PREF,
+    BLIS_GEMMSUP_CCR_UKR_ROW_PREF, BLIS_GEMMSUP_CCC_UKR_ROW_PREF,
+    BLIS_GEMMSUP_XXX_UKR_ROW_PREF,
+    BLIS_NUM_UKR_PREFS,
+    BLIS_UKR_PREFS_END_ = BLIS_VA_END
+} ukr_pref_t;
+
+typedef enum kimpl_e {
+    BLIS_REFERENCE_UKERNEL = 0, BLIS_VIRTUAL_UKERNEL, BLIS_OPTIMIZED_UKERNEL,
+    BLIS_NOTAPPLIC_UKERNEL, BLIS_NUM_UKR_IMPL_TYPES
+} kimpl_t;
+
+typedef enum bszid_e {
+    BLIS_KR, BLIS_MR, BLIS_NR, BLIS_MC, BLIS_KC, BLIS_NC,
+    BLIS_BBM, BLIS_BBN,
+    BLIS_M2, BLIS_N2,
+    BLIS_AF, BLIS_DF, BLIS_XF,
+    BLIS_MT, BLIS_NT, BLIS_KT,
+    BLIS_KR_SUP, BLIS_MR_SUP, BLIS_NR_SUP, BLIS_MC_SUP, BLIS_KC_SUP, BLIS_NC_SUP,
+    BLIS_NO_PART, BLIS_NUM_BLKSZS = BLIS_NO_PART,
+    BLIS_BLKSZS_END_ = BLIS_VA_END
+} bszid_t;
+
+/* The one architecture this library targets. */
+typedef enum arch_e { BLIS_ARCH_GFX950 = 0, BLIS_NUM_ARCHS } arch_t;
+
+/* Blocksize: default (v) and maximum (e) value per datatype, indexed
+   float, scomplex, double, dcomplex. */
+typedef struct blksz_s {
+    dim_t v[ BLIS_NUM_FP_TYPES ];
+    dim_t e[ BLIS_NUM_FP_TYPES ];
+} blksz_t;
+
+typedef struct mbool_s { bool v[ BLIS_NUM_FP_TYPES ]; } mbool_t;
+
+typedef struct auxinfo_s auxinfo_t;
+
+/* A context: blocksizes, their multiples, kernels and kernel preferences of
+   one architecture and induced method (bli_gks_query_cntx). */
+typedef struct cntx_s {
+    blksz_t blkszs[ BLIS_NUM_BLKSZS ];
+    bszid_t bmults[ BLIS_NUM_BLKSZS ];
+    void_fp ukrs[ BLIS_NUM_UKRS ][ BLIS_NUM_FP_TYPES ];
+    void_fp ukr2s[ BLIS_NUM_UKR2S ][ BLIS_NUM_FP_TYPES ];
+    mbool_t ukr_prefs[ BLIS_NUM_UKR_PREFS ];
+    ind_t   method;
+} cntx_t;
+
+/* Runtime objects are accepted for signature compatibility and ignored:
+   parallelism is chosen by the GPU launch planner. */
 typedef struct rntm_s rntm_t;
+
 
 #define BLIS_OBJECT_INITIALIZER \
 { NULL, { 0, 0 }, { 0, 0 }, 0, \
@@ -369,6 +463,7 @@ static inline prec_t bli_obj_comp_prec( const obj_t* o )
 static inline void bli_obj_set_comp_prec( prec_t p, obj_t* o )
 { o->info = ( objbits_t )( ( o->info & ~BLIS_COMP_PREC_BIT ) | ( ( objbits_t )p << BLIS_COMP_PREC_SHIFT ) ); }
 static inline bool bli_obj_is_complex( const obj_t* o ) { return ( o->info & BLIS_DOMAIN_BIT ) != 0; }
+static inline bool bli_dt_is_complex( num_t dt ) { return ( ( unsigned )dt & BLIS_DOMAIN_BIT ) != 0 && dt != BLIS_CONSTANT; }
 
 /* ---- initialization / information (frame/base/bli_init.c:40-97) ------- */
 
@@ -393,6 +488,77 @@ void bli_obj_scalar_init_detached( num_t dt, obj_t* beta );
 void bli_obj_free( obj_t* obj );
 void bli_setsc( double zeta_r, double zeta_i, obj_t* chi );
 void bli_getsc( const obj_t* chi, double* zeta_r, double* zeta_i );
+
+/* ---- kernel/blocksize registry (frame/base/bli_gks.c:58-609,
+   frame/base/bli_cntx.h:73-204, frame/base/bli_info.c) ----------------------
+   The registered context describes gfx950: MR x NR is the register block one
+   wave owns (the wave tile of the default large workgroup tile), MC x NC the
+   workgroup tile, KC its k slab, KR the MFMA k step, and MT/NT/KT the
+   thresholds of the single-call small-problem path. The gemm microkernel
+   entries take BLIS's packed micro-panels (A: MR x k, column-major, ld MR;
+   B: k x NR, row-major, ld NR) and m <= MR, n <= NR. */
+const cntx_t*  bli_gks_query_cntx( void );
+const cntx_t*  bli_gks_query_nat_cntx( void );
+const cntx_t*  bli_gks_query_ind_cntx( ind_t ind );
+kimpl_t        bli_gks_l3_ukr_impl_type( ukr_t ukr, ind_t method, num_t dt );
+const char*    bli_gks_l3_ukr_impl_string( ukr_t ukr, ind_t method, num_t dt );
+arch_t         bli_arch_query_id( void );
+const char*    bli_arch_string( arch_t id );
+const blksz_t* bli_cntx_get_blksz( bszid_t bs_id, const cntx_t* cntx );
+dim_t          bli_cntx_get_blksz_def_dt( num_t dt, bszid_t bs_id, const cntx_t* cntx );
+dim_t          bli_cntx_get_blksz_max_dt( num_t dt, bszid_t bs_id, const cntx_t* cntx );
+dim_t          bli_cntx_get_bmult_dt( num_t dt, bszid_t bs_id, const cntx_t* cntx );
+void_fp        bli_cntx_get_ukr_dt( num_t dt, ukr_t ukr_id, const cntx_t* cntx );
+bool           bli_cntx_get_ukr_prefs_dt( num_t dt, ukr_pref_t ukr_pref_id, const cntx_t* cntx );
+bool           bli_cntx_ukr_prefers_rows_dt( num_t dt, ukr_t ukr_id, const cntx_t* cntx );
+bool           bli_cntx_ukr_prefers_cols_dt( num_t dt, ukr_t ukr_id, const cntx_t* cntx );
+bool           bli_cntx_l3_sup_thresh_is_met( num_t dt, dim_t m, dim_t n, dim_t k,
+                                              const cntx_t* cntx );
+
+void bli_sgemm_ukernel( dim_t m, dim_t n, dim_t k, const float* alpha, const float* a,
+                        const float* b, const float* beta, float* c, inc_t rs_c, inc_t cs_c,
+                        auxinfo_t* data, const cntx_t* cntx );
+void bli_dgemm_ukernel( dim_t m, dim_t n, dim_t k, const double* alpha, const double* a,
+                        const double* b, const double* beta, double* c, inc_t rs_c, inc_t cs_c,
+                        auxinfo_t* data, const cntx_t* cntx );
+void bli_cgemm_ukernel( dim_t m, dim_t n, dim_t k, const scomplex* alpha, const scomplex* a,
+                        const scomplex* b, const scomplex* beta, scomplex* c, inc_t rs_c,
+                        inc_t cs_c, auxinfo_t* data, const cntx_t* cntx );
+void bli_zgemm_ukernel( dim_t m, dim_t n, dim_t k, const dcomplex* alpha, const dcomplex* a,
+                        const dcomplex* b, const dcomplex* beta, dcomplex* c, inc_t rs_c,
+                        inc_t cs_c, auxinfo_t* data, const cntx_t* cntx );
+
+gint_t      bli_info_get_int_type_size( void );
+const char* bli_info_get_int_type_size_str( void );
+gint_t      bli_info_get_num_fp_types( void );
+gint_t      bli_info_get_max_type_size( void );
+gint_t      bli_info_get_page_size( void );
+gint_t      bli_info_get_simd_size( void );
+gint_t      bli_info_get_simd_num_registers( void );
+gint_t      bli_info_get_simd_align_size( void );
+gint_t      bli_info_get_stack_buf_max_size( void );
+gint_t      bli_info_get_stack_buf_align_size( void );
+gint_t      bli_info_get_heap_addr_align_size( void );
+gint_t      bli_info_get_enable_blas( void );
+gint_t      bli_info_get_enable_cblas( void );
+gint_t      bli_info_get_blas_int_type_size( void );
+gint_t      bli_info_get_enable_threading( void );
+gint_t      bli_info_get_enable_openmp( void );
+gint_t      bli_info_get_enable_pthreads( void );
+gint_t      bli_info_get_enable_sandbox( void );
+const char* bli_info_get_gemm_ukr_impl_string( ind_t method, num_t dt );
+const char* bli_info_get_gemmtrsm_l_ukr_impl_string( ind_t method, num_t dt );
+const char* bli_info_get_gemmtrsm_u_ukr_impl_string( ind_t method, num_t dt );
+const char* bli_info_get_trsm_l_ukr_impl_string( ind_t method, num_t dt );
+const char* bli_info_get_trsm_u_ukr_impl_string( ind_t method, num_t dt );
+const char* bli_info_get_gemm_impl_string( num_t dt );
+const char* bli_info_get_gemmt_impl_string( num_t dt );
+const char* bli_info_get_symm_impl_string( num_t dt );
+const char* bli_info_get_syrk_impl_string( num_t dt );
+const char* bli_info_get_syr2k_impl_string( num_t dt );
+const char* bli_info_get_trmm_impl_string( num_t dt );
+const char* bli_info_get_trmm3_impl_string( num_t dt );
+const char* bli_info_get_trsm_impl_string( num_t dt );
 
 /* ---- level-3: typed API (frame/3/bli_l3_tapi.h:44-58) ----------------- */
 

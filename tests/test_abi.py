@@ -175,3 +175,55 @@ int main(void) {{ double a[4] = {{0}}; {call}; return 0; }}
     p = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert p.returncode == 255, (p.returncode, p.stderr)
     assert f"Parameter {param} to routine {routine} was incorrect" in p.stderr, p.stderr
+
+
+def test_registry_queries(tmp_path):
+    """bli_gks_query_cntx and the bli_cntx_* queries from a C program built
+    against include/blis.h (no device calls): the registered blocksizes are
+    the tile table's (FP64 tile 6: 128 x 128 workgroups of 64 x 32 waves, BK
+    16; FP32 tile 12: 256 x 128 of 64 x 64, BK 16), the gemm microkernel entry
+    is bli_?gemm_ukernel, and the small-problem thresholds bound the sup
+    path (frame/base/bli_cntx.h:186-193)."""
+    src = tmp_path / "reg.c"
+    src.write_text(r'''#include <stdio.h>
+#include "blis.h"
+int main(void) {
+    const cntx_t* c = bli_gks_query_cntx();
+    const num_t dts[4] = { BLIS_FLOAT, BLIS_SCOMPLEX, BLIS_DOUBLE, BLIS_DCOMPLEX };
+    const bszid_t ids[6] = { BLIS_KR, BLIS_MR, BLIS_NR, BLIS_MC, BLIS_KC, BLIS_NC };
+    for (int d = 0; d < 4; ++d) {
+        for (int i = 0; i < 6; ++i)
+            printf("%ld ", (long)bli_cntx_get_blksz_def_dt(dts[d], ids[i], c));
+        printf("| %ld %ld\n", (long)bli_cntx_get_blksz_max_dt(dts[d], BLIS_MC, c),
+               (long)bli_cntx_get_bmult_dt(dts[d], BLIS_MC, c));
+    }
+    printf("ukr %d %d\n", bli_cntx_get_ukr_dt(BLIS_DOUBLE, BLIS_GEMM_UKR, c) == (void_fp)bli_dgemm_ukernel,
+           bli_cntx_get_ukr_dt(BLIS_FLOAT, BLIS_GEMM_UKR, c) == (void_fp)bli_sgemm_ukernel);
+    printf("rows %d\n", (int)bli_cntx_ukr_prefers_rows_dt(BLIS_DOUBLE, BLIS_GEMM_UKR, c));
+    printf("sup %d %d\n", (int)bli_cntx_l3_sup_thresh_is_met(BLIS_DOUBLE, 64, 64, 64, c),
+           (int)bli_cntx_l3_sup_thresh_is_met(BLIS_DOUBLE, 4096, 4096, 4096, c));
+    printf("impl %s %s %s %s\n", bli_info_get_gemm_impl_string(BLIS_DOUBLE),
+           bli_info_get_gemm_impl_string(BLIS_DCOMPLEX),
+           bli_info_get_gemm_ukr_impl_string(BLIS_NAT, BLIS_DOUBLE),
+           bli_arch_string(bli_arch_query_id()));
+    printf("ind %d\n", bli_gks_query_ind_cntx(BLIS_1M) == c);
+    printf("info %ld %ld %ld\n", (long)bli_info_get_int_type_size(),
+           (long)bli_info_get_blas_int_type_size(), (long)bli_info_get_enable_cblas());
+    return 0;
+}
+''')
+    exe = tmp_path / "reg"
+    subprocess.run(["gcc", "-o", str(exe), str(src), f"-I{ROOT / 'include'}",
+                    f"-L{LIB.parent}", "-lblis_amd", f"-Wl,-rpath,{LIB.parent}"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True,
+                         timeout=60).stdout.splitlines()
+    assert out[0] == "4 64 64 256 16 128 | 256 64"      # float: tile 12
+    assert out[2] == "4 64 32 128 16 128 | 128 64"      # double: tile 6
+    assert out[1].startswith("4 32 64 128 8 128")      # scomplex: 1m on the float tile
+    assert out[3].startswith("4 32 32 64 8 128")       # dcomplex: 1m on the double tile
+    assert out[4] == "ukr 1 1"
+    assert out[5] == "rows 0"
+    assert out[6] == "sup 1 0"
+    assert out[7] == "impl native 1m optimzd gfx950"
+    assert out[8] == "ind 1"
+    assert out[9] == "info 64 32 1"

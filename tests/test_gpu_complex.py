@@ -190,9 +190,11 @@ def test_zgemm_full_size_probe(blis, dtype):
     alpha, beta = 0.7 - 0.2j, 0.5 + 0.5j
     ch = "z" if np.dtype(dtype) == np.complex128 else "c"
     say(f"{ch}gemm {n}^3 probe residual")
-    getattr(blis, f"bli_{ch}gemm")(0, 0, n, n, n, C.byref(_sx(dtype, alpha)), a.data_ptr(), 1, n,
-                                   b.data_ptr(), 1, n, C.byref(_sx(dtype, beta)), c.data_ptr(),
-                                   1, n)
+    i64, vp = C.c_int64, C.c_void_p
+    getattr(blis, f"bli_{ch}gemm")(0, 0, i64(n), i64(n), i64(n), C.byref(_sx(dtype, alpha)),
+                                   vp(a.data_ptr()), i64(1), i64(n), vp(b.data_ptr()), i64(1),
+                                   i64(n), C.byref(_sx(dtype, beta)), vp(c.data_ptr()), i64(1),
+                                   i64(n))
     torch.cuda.synchronize()
     A, B = a.cpu().numpy().T.astype(np.complex128), b.cpu().numpy().T.astype(np.complex128)
     Cn, C0 = c.cpu().numpy().T.astype(np.complex128), c0.cpu().numpy().T.astype(np.complex128)
