@@ -128,6 +128,8 @@ def _sig(lib):
     for ch in ("z", "c"):
         s(f"bli_{ch}gemm", None, C.c_int, C.c_int, dim_t, dim_t, dim_t, P, P, inc_t, inc_t, P,
           inc_t, inc_t, P, P, inc_t, inc_t)
+    s("bli_amd_gemm_noalloc", C.c_int, C.c_int, C.c_int, C.c_int, dim_t, dim_t, dim_t, P, P,
+      inc_t, inc_t, P, inc_t, inc_t, P, P, inc_t, inc_t, P)
     s("bli_amd_gemm_batch_plan_execute", None, P, C.c_int)
     s("bli_amd_gemm_batch_plan_destroy", None, P)
     s("bli_amd_gemm_batch_plan_work_items", C.c_int, P)

@@ -465,6 +465,40 @@ void gemm_strided_run(const GemmDesc& d, int count, int64_t sa, int64_t sb, int6
     }
 }
 
+// ---------------------------------------------------------------------------
+// GEMM without allocation (bli_amd_gemm_noalloc)
+// ---------------------------------------------------------------------------
+//
+// The device memory a GEMM may take beyond its operands (packed copies of
+// general-stride operands, split-k partitions) comes from one fixed array
+// per GPU instead of the workspace arena. In host code `scratch` only
+// names the array; its device address on the current GPU comes from
+// hipGetSymbolAddress, taken once per device and kept in g_scratch_ptr.
+
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+static std::mutex g_scratch_mutex;
+
+// Device address of `scratch` on the current GPU (nullptr on failure).
+static void* scratch_address(int dev) {
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
 } // namespace blis_amd
 
 extern "C" __attribute__((visibility("default")))
@@ -485,4 +519,68 @@ int bli_amd_gemm_tile_count(int is_double) {
 extern "C" __attribute__((visibility("default")))
 int bli_amd_gemm_default_tile(int is_double, long long m, long long n, long long k) {
     return blis_amd::gemm_default_tile(is_double ? blis_amd::Dt::F64 : blis_amd::Dt::F32, m, n, k);
+}
+
+// C := beta*C + alpha*op(A)*op(B) for FP32/FP64 device operands, with no
+// device allocation: scratch comes from the library's fixed per-GPU array
+// (64 MiB). The kernels and their configuration are those of bli_?gemm with
+// the default (heuristic) selector, so the results are the same bits.
+extern "C" __attribute__((visibility("default")))
+int bli_amd_gemm_noalloc(num_t dt, trans_t transa, trans_t transb, dim_t m, dim_t n, dim_t k,
+                         const void* alpha, const void* a, inc_t rs_a, inc_t cs_a,
+                         const void* b, inc_t rs_b, inc_t cs_b, const void* beta, void* c,
+                         inc_t rs_c, inc_t cs_c, void* stream) {
+    using namespace blis_amd;
+    if (dt != BLIS_FLOAT && dt != BLIS_DOUBLE) return BLIS_INVALID_DATATYPE;
+    auto valid = [](trans_t t) {
+        return t == BLIS_NO_TRANSPOSE || t == BLIS_TRANSPOSE || t == BLIS_CONJ_NO_TRANSPOSE ||
+               t == BLIS_CONJ_TRANSPOSE;
+    };
+    if (!valid(transa) || !valid(transb)) return BLIS_INVALID_TRANS;
+    if (m < 0 || n < 0 || k < 0) return BLIS_NEGATIVE_DIMENSION;
+    if (alpha == nullptr || beta == nullptr) return BLIS_NULL_POINTER;
+    if (m == 0 || n == 0) return BLIS_SUCCESS;
+    const Dt d = dt == BLIS_DOUBLE ? Dt::F64 : Dt::F32;
+    const double al = d == Dt::F64 ? *(const double*)alpha : (double)*(const float*)alpha;
+    const double be = d == Dt::F64 ? *(const double*)beta : (double)*(const float*)beta;
+    const bool touch_ab = k > 0 && al != 0.0;
+    // Operands must already be in device memory (host staging allocates).
+    if (c == nullptr || !is_device_pointer(c)) return BLIS_FAILURE;
+    if (touch_ab && (a == nullptr || b == nullptr || !is_device_pointer(a) ||
+                     !is_device_pointer(b)))
+        return BLIS_FAILURE;
+    const bool ta = (unsigned)transa & BLIS_TRANS_BIT, tb = (unsigned)transb & BLIS_TRANS_BIT;
+    GemmDesc g;
+    g.dt = d; g.m = m; g.n = n; g.k = k;
+    g.a = { const_cast<void*>(a), ta ? cs_a : rs_a, ta ? rs_a : cs_a };
+    g.b = { const_cast<void*>(b), tb ? cs_b : rs_b, tb ? rs_b : cs_b };
+    g.c = { c, rs_c, cs_c };
+    g.alpha = al; g.beta = be;
+    hipStream_t st = (hipStream_t)stream;
+    if (!touch_ab) {
+        scalm_run(d, m, n, be, g.c, st);
+        return BLIS_SUCCESS;
+    }
+    Plan pl = layout(g);
+    GemmConfig cfg;
+    cfg.tile = gemm_default_tile(d, pl.m, pl.n, pl.k);
+    cfg.split_k = gemm_default_split(pl.m, pl.n, pl.k);
+    // Scratch this call needs, carved as Scratch::take does (256-byte
+    // aligned pieces): packed general-stride operands and split-k slices.
+    const size_t es = dt_size(d);
+    size_t need = 0;
+    if (pl.a_packed) need += align256((size_t)pl.m * pl.k * es);
+    if (pl.b_packed) need += align256((size_t)pl.k * pl.n * es);
+    const SplitK sk = split_params(pl.k, cfg.split_k);
+    if (sk.parts > 1) need += align256((size_t)sk.parts * pl.m * pl.n * es);
+    if (need > NOALLOC_SCRATCH_BYTES) return BLIS_INSUFFICIENT_STACK_BUF_SIZE;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return BLIS_FAILURE;
+    Scratch sc;
+    sc.base = (char*)scratch_address(dev);
+    sc.cap = NOALLOC_SCRATCH_BYTES;
+    if (sc.base == nullptr) return BLIS_FAILURE;
+    pack_operands(pl, d, sc, st);
+    launch_plan(d, pl, cfg, al, be, sc, st);
+    return BLIS_SUCCESS;
 }

@@ -37,6 +37,27 @@ void bli_amd_gemm_set_config( int tile, int group_m, int xcd_remap, int split_k 
 int  bli_amd_gemm_tile_count( int is_double );
 int  bli_amd_gemm_default_tile( int is_double, long long m, long long n, long long k );
 
+/* ---- GEMM without device allocation -------------------------------------
+   No counterpart in the reference. For callers that cannot allocate device
+   memory: code under HIP graph capture, serving loops, harnesses that
+   forbid hipMalloc. C := beta*C + alpha*op(A)*op(B) for dt BLIS_FLOAT or
+   BLIS_DOUBLE, with the same kernels and default configuration as
+   bli_?gemm (the results are the same bits). A, B and C must be device
+   memory; alpha and beta are host pointers. The call is asynchronous on
+   `stream` (a hipStream_t; NULL is the null stream) and allocates nothing:
+   the scratch it needs (split-k partitions, packed copies of general-stride
+   operands) comes from one fixed 64 MiB array per GPU that the library
+   holds until the process ends. Every call on a GPU shares that array, so
+   calls that overlap on one GPU race. Returns BLIS_SUCCESS;
+   BLIS_INSUFFICIENT_STACK_BUF_SIZE when the problem needs more scratch than
+   the array holds (nothing is run); BLIS_FAILURE for host operands;
+   BLIS_INVALID_DATATYPE, BLIS_INVALID_TRANS, BLIS_NEGATIVE_DIMENSION or
+   BLIS_NULL_POINTER for invalid arguments. */
+int bli_amd_gemm_noalloc( num_t dt, trans_t transa, trans_t transb, dim_t m, dim_t n, dim_t k,
+                          const void* alpha, const void* a, inc_t rs_a, inc_t cs_a,
+                          const void* b, inc_t rs_b, inc_t cs_b, const void* beta,
+                          void* c, inc_t rs_c, inc_t cs_c, void* stream );
+
 /* ---- batched GEMM plans ------------------------------------------------
    A plan holds a batch (same arguments as bli_?gemm_batch, operands already
    in device memory) as device descriptors and an LPT-ordered work list. Each
