@@ -12,7 +12,9 @@ semantics). The JSON line reports:
   cpu_baseline the oracle's five-loop DGEMM timed on the host cores on a
               bounded 2048^3 sample of the same workload;
   extras      the other configured shapes (SGEMM, skinny k = 256, DTRSM,
-              DGEMV, DAXPYV, DDOTV, batched uneven GEMM) at a few reps each.
+              DGEMV, DAXPYV, DDOTV, batched uneven GEMM) at a few reps each,
+              small-call latency (config 1, 512^3) and the structured and
+              complex operations.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu]
 """
@@ -194,6 +196,14 @@ def extras(bench: Bench):
         del fn
         torch.cuda.empty_cache()
 
+    # Small single calls (the sup path) including config 1 (512^3, alpha 1,
+    # beta 0.5): synchronous per-call latency against the library's own
+    # launch-and-wait floor (a one-element axpyv), and device time per call.
+    out.update(sup_latency(bench, log))
+
+    # Structured level-3 and complex / mixed-datatype GEMM at scale.
+    out.update(struc_bench(bench, rec))
+
     # DTRSM left/lower/no-trans/non-unit, m = 4096, n = 1024.
     m, n = 4096, 1024
     rec("dtrsm_llnn_4096x1024", trsm_ms(bench), float(m) * m * n, "TFLOP/s", PEAK_TFLOPS["f64"],
@@ -242,6 +252,82 @@ def extras(bench: Bench):
                                    "heuristic": j["heuristic_holdout"],
                                    "shapes": j["shapes"], "holdout": j["holdout"]}
     return out
+
+
+def sup_latency(bench, log, reps=200):
+    """Per-call latency of small DGEMMs (config 1 is 512^3 NN, alpha 1,
+    beta 0.5): `sync_us` is the host wall time of one synchronous call
+    (BLIS semantics) averaged over `reps` calls; `device_us` the device time
+    per call with the calls enqueued back to back; `floor_us` the same
+    synchronous wall time for a one-element axpyv (one launch and one wait
+    through the library)."""
+    torch = bench.torch
+    L = bench.L
+    out = {}
+    x = bench.uniform(8, torch.float64)
+    y = bench.uniform(8, torch.float64)
+    one = C.c_double(1.0)
+
+    def wall_us(fn):
+        for _ in range(20):
+            fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) * 1e6 / reps
+
+    floor = wall_us(lambda: L.bli_daxpyv(0, 1, C.byref(one), x.data_ptr(), 1, y.data_ptr(), 1))
+    for s in (64, 128, 256, 512):
+        fn = bench.gemm_fn("d", 0, 0, s, s, s)
+        sync_us = wall_us(fn)
+        dev_us = bench.kernel_ms(fn, reps) * 1e3
+        key = "dgemm_config1_512" if s == 512 else f"dgemm_small_{s}"
+        out[key] = {"sync_us": round(sync_us, 2), "device_us": round(dev_us, 2),
+                    "floor_us": round(floor, 2),
+                    "TFLOP/s_device": round(2.0 * s ** 3 / (dev_us * 1e-6) / 1e12, 3)}
+        log(f"  {key:34s} sync {sync_us:7.2f} us  device {dev_us:7.2f} us  "
+            f"(launch+wait floor {floor:.2f} us)")
+        del fn
+    return out
+
+
+def struc_bench(bench, rec, n=8192):
+    """gemmt/syrk (n^2 k flops), trmm (n^2 m), symm (2 n^2 m) at n = 8192 in
+    FP64, and complex ZGEMM 4096^3 (8 mnk real flops) against the FP64 peak."""
+    torch = bench.torch
+    L = bench.L
+    dt = torch.float64
+    a = bench.uniform(n * n, dt)
+    b = bench.uniform(n * n, dt)
+    c = bench.uniform(n * n, dt)
+    one, half = C.c_double(1.0), C.c_double(0.5)
+    P = PEAK_TFLOPS["f64"]
+    ms = bench.kernel_ms(lambda: L.bli_dsyrk(0xC0, 0, n, n, C.byref(one), a.data_ptr(), 1, n,
+                                             C.byref(half), c.data_ptr(), 1, n), 3)
+    rec(f"dsyrk_l_{n}", ms, float(n) * n * n, "TFLOP/s", P, "mfma")
+    ms = bench.kernel_ms(lambda: L.bli_dsymm(0, 0xC0, 0, 0, n, n, C.byref(one), a.data_ptr(), 1, n,
+                                             b.data_ptr(), 1, n, C.byref(half), c.data_ptr(), 1,
+                                             n), 3)
+    rec(f"dsymm_ll_{n}", ms, 2.0 * n * n * n, "TFLOP/s", P, "mfma")
+    ms = bench.kernel_ms(lambda: L.bli_dtrmm(0, 0xC0, 0, 0, n, n, C.byref(one), a.data_ptr(), 1,
+                                             n, b.data_ptr(), 1, n), 3)
+    rec(f"dtrmm_llnn_{n}", ms, float(n) * n * n, "TFLOP/s", P, "mfma")
+    del a, b, c
+    torch.cuda.empty_cache()
+    z = 4096
+    za = bench.uniform(2 * z * z, dt)
+    zb = bench.uniform(2 * z * z, dt)
+    zc = bench.uniform(2 * z * z, dt)
+    al = (C.c_double * 2)(0.5, 0.25)
+    be = (C.c_double * 2)(1.0, 0.0)
+    f = L.bli_zgemm
+    ms = bench.kernel_ms(lambda: f(0, 0, z, z, z, C.cast(al, C.c_void_p), za.data_ptr(), 1, z,
+                                   zb.data_ptr(), 1, z, C.cast(be, C.c_void_p), zc.data_ptr(), 1,
+                                   z), 3)
+    rec(f"zgemm_nn_{z}", ms, 8.0 * z * z * z, "TFLOP/s", P, "mfma")
+    del za, zb, zc
+    torch.cuda.empty_cache()
+    return {}
 
 
 def split_bench(bench, log, size=8192, sample_rows=8):

@@ -55,6 +55,7 @@ SOURCES = [
     "api_complex.cpp",
     "cblas.cpp",
     "cntx.cpp",
+    "sup.hip",
 ]
 
 

@@ -132,4 +132,9 @@ inline TileProblem<T> make_problem(const Plan& pl, double alpha, double beta) {
     return p;
 }
 
+// The single-call small-problem path (sup.hip): applies to m, n, k all at
+// most SMALL_GEMM_MAX_DIM; the plan's operands must have a unit stride.
+bool gemm_sup_applies(int64_t m, int64_t n, int64_t k);
+void gemm_sup_run(Dt dt, const Plan& pl, double alpha, double beta, hipStream_t st);
+
 } // namespace blis_amd

@@ -23,8 +23,8 @@ namespace blis_amd {
 enum class Dt : int { F32 = 0, F64 = 1 };
 
 // Largest m, n and k of the single-call small-problem path (BLIS's
-// MT/NT/KT thresholds, frame/base/bli_cntx.h:186-193).
-constexpr int64_t SMALL_GEMM_MAX_DIM = 256;
+// MT/NT/KT thresholds, frame/base/bli_cntx.h:186-193; sup.hip).
+constexpr int64_t SMALL_GEMM_MAX_DIM = 512;
 
 inline size_t dt_size(Dt d) { return d == Dt::F64 ? 8 : 4; }
 
@@ -176,6 +176,7 @@ BLIS_AMD_HIDDEN void md_accum_run(Dt pdt, bool p_cx, const void* p, int64_t ldp,
 // ---- configuration selection (select.hip) ----
 enum { SELECT_HEURISTIC = 0, SELECT_LEARNED = 1, SELECT_EXHAUSTIVE = 2 };
 BLIS_AMD_HIDDEN GemmConfig config_of(int cfg);
+BLIS_AMD_HIDDEN int selector_mode();
 BLIS_AMD_HIDDEN int selector_pick(Dt dt, int64_t m, int64_t n, int64_t k, bool a_k, bool b_k,
                                   bool va, bool vb, bool vc);
 BLIS_AMD_HIDDEN void gemm_features(Dt dt, int64_t m, int64_t n, int64_t k, bool a_k, bool b_k,

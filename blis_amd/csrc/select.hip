@@ -247,6 +247,11 @@ static int heuristic_cfg(Dt dt, int64_t m, int64_t n, int64_t k) {
     return gemm_default_tile(dt, m, n, k) * GEMM_NSCHED;
 }
 
+int selector_mode() {
+    selector_init();
+    return SS().mode;
+}
+
 int selector_pick(Dt dt, int64_t m, int64_t n, int64_t k, bool a_k, bool b_k, bool va, bool vb,
                   bool vc) {
     selector_init();

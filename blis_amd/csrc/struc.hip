@@ -40,6 +40,7 @@ struct StrucSpec {
     int64_t cdoff = 0;     // its diagonal offset
     int     kmode = K_FULL;
     int64_t koff = 0;
+    int     reverse = 0;   // walk the tiles last to first
 };
 
 // The same problem after the operation transpose C^T = B^T A^T.
@@ -69,7 +70,8 @@ __global__ __launch_bounds__(TL::NT) void gemm_struc_kernel(TileProblem<T> p, Gr
                                                             StrucSpec s) {
     __shared__ T smem[2 * TL::STAGE];
     int tm, tn;
-    tile_coords(blockIdx.x, gridDim.x, g, tm, tn);
+    const int bid = s.reverse ? (int)gridDim.x - 1 - (int)blockIdx.x : (int)blockIdx.x;
+    tile_coords(bid, gridDim.x, g, tm, tn);
     const int64_t m0 = (int64_t)tm * TL::BM, n0 = (int64_t)tn * TL::BN;
     if constexpr (MASK) {
         // lower: stored rows of column j are i >= j - doff; upper: i <= j - doff.
@@ -254,9 +256,17 @@ void gemm_struc_run(const GemmDesc& d, StrucSpec s, hipStream_t st) {
     Plan pl = layout(d);
     Scratch sc;
     pack_operands(pl, d.dt, sc, st);
+    // The work per tile varies across the grid (tiles outside C's triangle
+    // do nothing; a triangular operand's k range grows along the rows or the
+    // columns), so the tiles are dealt round-robin over the XCDs instead of
+    // in contiguous ranges, which would give some XCDs most of the work.
+    // Tiles are numbered by groups of 8 tile rows, columns in order within a
+    // group; when the k range grows with the row or the column index the
+    // grid runs last tile first, so the longest tiles start first.
     GridInfo g{};
     g.group_m = 8;
-    g.xcd_remap = 1;
+    g.xcd_remap = 0;
+    s.reverse = (s.kmode == K_ROW_END || s.kmode == K_COL_END) ? 1 : 0;
     const int tile = gemm_default_tile(d.dt, pl.m, pl.n, pl.k);
     if (d.dt == Dt::F64)
         launch_struc<double>(tile, make_problem<double>(pl, d.alpha, d.beta), pl.a_k, pl.b_k, g,
