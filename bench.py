@@ -415,7 +415,7 @@ def split_bench(bench, log, size=8192, sample_rows=8):
     return out
 
 
-def batch_bench(bench, ch, key, log, count=10000, seed=42):
+def batch_bench(bench, ch, key, log, count=10000, seed=42, replays=None):
     import numpy as np
     torch = bench.torch
     L = bench.L
@@ -454,6 +454,14 @@ def batch_bench(bench, ch, key, log, count=10000, seed=42):
     gs = I(*([1] * count))
     cargs = [C.cast(x, C.c_void_p) for x in args] + [count, C.cast(gs, C.c_void_p)]
     plan = getattr(L, f"bli_amd_{ch}gemm_batch_plan_create")(*cargs)
+    if replays is not None:
+        # Plan replays in the default mode only (counter passes).
+        for _ in range(replays):
+            L.bli_amd_gemm_batch_plan_execute(plan, 1)
+        L.bli_amd_gemm_batch_plan_destroy(plan)
+        del A, B, Cm
+        torch.cuda.empty_cache()
+        return {}
     modes = {}
     for mode in range(9):
         L.bli_amd_gemm_batch_plan_set_mode(plan, mode)
@@ -480,6 +488,68 @@ def batch_bench(bench, ch, key, log, count=10000, seed=42):
     return res
 
 
+def kernel_probe(bench):
+    """Two calls of each hot kernel on its bench workload, for rocprofv3 --pmc
+    passes (one counter group per run): a 1 GiB device copy (the counter
+    calibration), DGEMM NN 8192^3, DTRSM L/L/N/N 4096 x 1024, DAXPYV and
+    DDOTV n = 2^28, DGEMV N and T 16384^2, split DGEMM 8192^3 (sigma 0) and
+    the 10,000-problem FP64 batch (plan replay)."""
+    torch = bench.torch
+    L = bench.L
+    src = bench.uniform(1 << 27, torch.float64)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    step = bench.gemm_fn("d", 0, 0, 8192, 8192, 8192)
+    step()
+    step()
+    torch.cuda.synchronize()
+    del step
+    torch.cuda.empty_cache()
+    log("kernel probe: dgemm done")
+    trsm_ms(bench)
+    log("kernel probe: dtrsm done")
+    n = 1 << 28
+    x = bench.uniform(n, torch.float64)
+    y = bench.uniform(n, torch.float64)
+    al = C.c_double(0.5)
+    rho = torch.zeros(1, dtype=torch.float64, device="cuda")
+    for _ in range(2):
+        L.bli_daxpyv(0, n, C.byref(al), x.data_ptr(), 1, y.data_ptr(), 1)
+    for _ in range(2):
+        L.bli_ddotv(0, 0, n, x.data_ptr(), 1, y.data_ptr(), 1, C.c_void_p(rho.data_ptr()))
+    torch.cuda.synchronize()
+    del x, y
+    torch.cuda.empty_cache()
+    m = 16384
+    a = bench.uniform(m * m, torch.float64)
+    x = bench.uniform(m, torch.float64)
+    y = bench.uniform(m, torch.float64)
+    one, half = C.c_double(1.0), C.c_double(0.5)
+    for t in (0, 0, 8, 8):
+        L.bli_dgemv(t, 0, m, m, C.byref(one), a.data_ptr(), 1, m, x.data_ptr(), 1, C.byref(half),
+                    y.data_ptr(), 1)
+    torch.cuda.synchronize()
+    del a, x, y
+    torch.cuda.empty_cache()
+    log("kernel probe: level-1/2 done")
+    s = 8192
+    a = bench.uniform(s * s, torch.float64)
+    b = bench.uniform(s * s, torch.float64)
+    c = torch.zeros(s * s, dtype=torch.float64, device="cuda")
+    eb = C.c_double()
+    zero = C.c_double(0.0)
+    L.bli_amd_dgemm_split(0, 0, s, s, s, C.byref(one), a.data_ptr(), 1, s, b.data_ptr(), 1, s,
+                          C.byref(zero), c.data_ptr(), 1, s, 0, C.byref(eb))
+    torch.cuda.synchronize()
+    del a, b, c
+    torch.cuda.empty_cache()
+    log("kernel probe: split done")
+    batch_bench(bench, "d", "f64", log, replays=2)
+    log("kernel probe: batch done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -493,6 +563,8 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one 1 GiB device copy, then two headline DGEMM calls (for "
                          "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE calibration)")
+    ap.add_argument("--kernel-probe", action="store_true",
+                    help="two calls of each hot kernel (for rocprofv3 --pmc counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -515,6 +587,9 @@ def main():
         for ch, key in (("d", "f64"), ("s", "f32")):
             res.update(batch_bench(bench, ch, key, log))
         print(json.dumps(res), flush=True)
+        return
+    if args.kernel_probe:
+        kernel_probe(bench)
         return
     if args.traffic_probe:
         # Calibration: a plain device copy of exactly 1 GiB read + 1 GiB written.

@@ -361,6 +361,10 @@ static void launch_plan(Dt dt, Plan& pl, const GemmConfig& cfg, double alpha, do
     // Split-k: each partition a full product into its own m x n slice of
     // scratch, then one reduction into C.
     SplitK sk = split_params(pl.k, cfg.split_k);
+    if (gemm_sup_deep_k(tile, sk.parts, pl.k)) {
+        gemm_sup_run(dt, pl, alpha, beta, st);
+        return;
+    }
     const size_t es = dt_size(dt);
     if (sk.parts > 1) sk.ws = sc.take((size_t)sk.parts * pl.m * pl.n * es, 17);
     if (dt == Dt::F64) {

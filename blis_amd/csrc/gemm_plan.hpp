@@ -133,8 +133,12 @@ inline TileProblem<T> make_problem(const Plan& pl, double alpha, double beta) {
 }
 
 // The single-call small-problem path (sup.hip): applies to m, n, k all at
-// most SMALL_GEMM_MAX_DIM; the plan's operands must have a unit stride.
+// most SMALL_GEMM_MAX_DIM with k > 64; the plan's operands must have a unit
+// stride. gemm_sup_deep_k: the 32 x 32 table tile over k > 64 in one
+// partition runs as the same kernel. BLIS_AMD_SUP=0 in the environment
+// turns both off (read once).
 bool gemm_sup_applies(int64_t m, int64_t n, int64_t k);
+bool gemm_sup_deep_k(int tile, int parts, int64_t k);
 void gemm_sup_run(Dt dt, const Plan& pl, double alpha, double beta, hipStream_t st);
 
 } // namespace blis_amd

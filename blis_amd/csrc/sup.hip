@@ -67,9 +67,26 @@ void launch_sup(const Plan& pl, double alpha, double beta, hipStream_t st) {
 // 16.3 us on the table's 32 x 32 tile, 256^3 7.6 against 9.6, 128^3 5.8
 // against 6.4. Up to k = 64 the table tile (one 16-deep slab in flight) is
 // as fast or faster (64^3: 4.6 against 4.9 us), so those calls keep it.
+static bool sup_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("BLIS_AMD_SUP");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 bool gemm_sup_applies(int64_t m, int64_t n, int64_t k) {
-    return m <= SMALL_GEMM_MAX_DIM && n <= SMALL_GEMM_MAX_DIM && k <= SMALL_GEMM_MAX_DIM &&
-           k > 64;
+    return sup_enabled() && m <= SMALL_GEMM_MAX_DIM && n <= SMALL_GEMM_MAX_DIM &&
+           k <= SMALL_GEMM_MAX_DIM && k > 64;
+}
+
+// Table tile 4 is the 32 x 32 tile of four 16 x 16 waves: the sup kernel
+// computes the same tiles with deeper slabs. Measured (FP64, column-major,
+// ld 4096): 512 x 1024 x 512 20.1 us against 21.3 us; at k = 1024 the
+// 64-deep slabs allow only two workgroups per CU and lose (1024^3: 64.2
+// against 57.8 us), so the substitution stops at k = 512.
+bool gemm_sup_deep_k(int tile, int parts, int64_t k) {
+    return sup_enabled() && tile == 4 && parts <= 1 && k > 64 && k <= SMALL_GEMM_MAX_DIM;
 }
 
 void gemm_sup_run(Dt dt, const Plan& pl, double alpha, double beta, hipStream_t st) {

@@ -367,13 +367,15 @@ void update(const Solve& s, int64_t dst0, int64_t dstn, int64_t src0, int64_t sr
     d.alpha = -1.0;
     d.beta = beta;
     // FP64 updates of 1024 and 2048 rows (n = 1024, k = rows, operands in
-    // the solve's layout) measured faster on 64x64x32 tiles (55 against
-    // 60 us) and on 128x64x32 tiles of eight waves (158-161 against 171-186
-    // us) than on the size heuristic's 32x32 and 64x64 tiles. Every tile
-    // gives the same result.
+    // the solve's layout), measured over the tile table with 1, 2 and 4 k
+    // partitions: 1024 rows are fastest on 128x64x16 tiles of eight waves
+    // in two partitions (53.1 us against 59.7 on 64x64x32 tiles and 57.8 on
+    // the size heuristic's 32x32), 2048 rows on 128x64x32 tiles in one
+    // (158 against 173 us on the heuristic's 64x64). Every tile gives the
+    // same result; the partitioned sum rounds in another order.
     GemmConfig cfg;
     if (s.dt == Dt::F64 && dstn >= 1536 && srcn >= 1536) cfg.tile = 11;
-    else if (s.dt == Dt::F64 && dstn >= 768 && srcn >= 768) cfg.tile = 8;
+    else if (s.dt == Dt::F64 && dstn >= 768 && srcn >= 768) { cfg.tile = 7; cfg.split_k = 2; }
     gemm_run(d, cfg.tile >= 0 ? &cfg : nullptr, s.st);
 }
 
