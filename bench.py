@@ -244,14 +244,67 @@ def extras(bench: Bench):
     # Split-precision DGEMM/SGEMM 8192^3 vs native at matched error.
     out.update(split_bench(bench, log))
 
-    # Learned tile selector: held-out regret from the shipped evaluation.
+    # Learned tile selector: regret measured now on held-out shapes, and the
+    # offline figure from the shipped evaluation.
+    out["selector_holdout"] = selector_online(bench, log)
     sel = ROOT / "blis_amd" / "data" / "selector_gfx950.json"
     if sel.exists():
         j = json.loads(sel.read_text())
-        out["selector_holdout"] = {"learned": j["learned_holdout"],
-                                   "heuristic": j["heuristic_holdout"],
-                                   "shapes": j["shapes"], "holdout": j["holdout"]}
+        out["selector_holdout"]["offline"] = {"learned": j["learned_holdout"],
+                                              "heuristic": j["heuristic_holdout"],
+                                              "shapes": j["shapes"], "holdout": j["holdout"]}
     return out
+
+
+def selector_online(bench, log, count=240, max_ms=2.0):
+    """Time regret of the default (learned) selector and of the size
+    heuristic against exhaustive autotuning, measured in this run: held-out
+    shapes of the two tuning databases (the last 10% of each, which the
+    network was not trained on) whose fastest configuration took at most
+    max_ms there, every 'stride'-th one up to `count` shapes. For each shape
+    all 42 configurations are timed (bli_amd_gemm_autotune) and the learned
+    pick (bli_amd_gemm_selected_config in the default mode) and the
+    heuristic's are priced with those times."""
+    import numpy as np
+    from blis_amd import tune
+    L = tune._setup(bench.L)
+    L.bli_amd_gemm_get_selector.restype = C.c_int
+    L.bli_amd_gemm_selected_config.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_int64] * 6
+    L.bli_amd_gemm_selected_config.restype = C.c_int
+    mode = L.bli_amd_gemm_get_selector()
+    data = ROOT / "blis_amd" / "data"
+    db = tune.load_db([data / "gemm_tuning_gfx950.npz", data / "gemm_tuning_round_gfx950.npz"],
+                      0.1)
+    ho = np.where(db["held"] & (db["ms"].min(axis=1) <= max_ms))[0]
+    stride = max(1, len(ho) // count)
+    rows = ho[::stride][:count]
+    ncfg = L.bli_amd_gemm_config_count(1)
+    got_l, got_h, best = [], [], []
+    t0 = time.perf_counter()
+    for i in rows:
+        m, n, k = (int(x) for x in db["dims"][i])
+        args = (int(db["dbl"][i]), 8 * int(db["ta"][i]), 8 * int(db["tb"][i]), m, n, k,
+                int(db["lda"][i]), int(db["ldb"][i]), int(db["ldc"][i]))
+        t = np.zeros(ncfg, dtype=np.float32)
+        if L.bli_amd_gemm_autotune(*args, t.ctypes.data_as(C.POINTER(C.c_float))) != 0:
+            raise RuntimeError(f"autotune rejected {args}")
+        pick = L.bli_amd_gemm_selected_config(*args)
+        heur = L.bli_amd_gemm_default_tile(args[0], m, n, k) * (ncfg // 14)
+        got_l.append(t[pick])
+        got_h.append(t[heur])
+        best.append(t.min())
+    got_l, got_h, best = np.array(got_l), np.array(got_h), np.array(best)
+    res = {"mode": {0: "heuristic", 1: "learned", 2: "exhaustive"}.get(mode, str(mode)),
+           "shapes": int(len(rows)), "seconds": round(time.perf_counter() - t0, 1),
+           "learned_total_regret": round(float(got_l.sum() / best.sum() - 1), 4),
+           "learned_geomean_regret": round(float(np.exp(np.mean(np.log(got_l / best))) - 1), 4),
+           "heuristic_total_regret": round(float(got_h.sum() / best.sum() - 1), 4),
+           "heuristic_geomean_regret": round(float(np.exp(np.mean(np.log(got_h / best))) - 1), 4)}
+    log(f"  selector (default mode {res['mode']}), {res['shapes']} held-out shapes timed now: "
+        f"learned regret {res['learned_total_regret']:.4f} total / "
+        f"{res['learned_geomean_regret']:.4f} geomean; heuristic "
+        f"{res['heuristic_total_regret']:.4f} / {res['heuristic_geomean_regret']:.4f}")
+    return res
 
 
 def sup_latency(bench, log, reps=200):
@@ -560,6 +613,7 @@ def main():
     ap.add_argument("--only-split", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--only-batch", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--only-trsm", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--only-selector", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one 1 GiB device copy, then two headline DGEMM calls (for "
                          "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE calibration)")
@@ -575,6 +629,9 @@ def main():
     bench = Bench()
     if args.only_split:
         print(json.dumps(split_bench(bench, log)), flush=True)
+        return
+    if args.only_selector:
+        print(json.dumps(selector_online(bench, log)), flush=True)
         return
     if args.only_trsm:
         ms = trsm_ms(bench)

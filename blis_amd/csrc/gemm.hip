@@ -398,11 +398,12 @@ void gemm_run(const GemmDesc& d, const GemmConfig* cfg, hipStream_t st) {
         return;
     }
     Plan pl = layout(d);
-    // Small problems with no configuration given take the sup path (the
-    // selector's own modes choose among the table's tiles instead).
+    // Small problems with no configuration given take the sup path, ahead
+    // of the heuristic and the learned selector (exhaustive selection times
+    // the table's tiles instead).
     if (cfg == &g_override && g_override.tile < 0 && g_override.split_k <= 1 &&
         !pl.a_packed && !pl.b_packed && gemm_sup_applies(pl.m, pl.n, pl.k) &&
-        selector_mode() == SELECT_HEURISTIC) {
+        selector_mode() != SELECT_EXHAUSTIVE) {
         gemm_sup_run(d.dt, pl, d.alpha, d.beta, st);
         return;
     }

@@ -375,7 +375,7 @@ bool MlpNet::save(const char* path) const {
     return ok;
 }
 
-bool MlpNet::load(const char* path) {
+bool mlp_read(const char* path, std::vector<float>& params, std::vector<float>& musd) {
     FILE* f = std::fopen(path, "rb");
     if (!f) return false;
     char magic[8];
@@ -387,8 +387,44 @@ bool MlpNet::load(const char* path) {
               std::fread(ms.data(), sizeof(float), ms.size(), f) == ms.size() &&
               std::fread(p.data(), sizeof(float), p.size(), f) == p.size();
     std::fclose(f);
-    if (ok) set_params(p.data(), ms.data());
+    if (ok) {
+        params.swap(p);
+        musd.swap(ms);
+    }
     return ok;
+}
+
+bool MlpNet::load(const char* path) {
+    std::vector<float> p, ms;
+    if (!mlp_read(path, p, ms)) return false;
+    set_params(p.data(), ms.data());
+    return true;
+}
+
+// The loops of normalise_kernel, forward_kernel and argmin_kernel for one
+// row, with the same fused multiply-adds in the same order.
+int MlpHost::argmin(const float* feats) const {
+    const float* P = p.data();
+    float x[IN], h1[H], h2[H];
+    for (int i = 0; i < IN; ++i) x[i] = (feats[i] - musd[i]) / musd[IN + i];
+    for (int j = 0; j < H; ++j) {
+        float a = P[OFF_B1 + j];
+        for (int i = 0; i < IN; ++i) a = std::fmaf(P[OFF_W1 + j * IN + i], x[i], a);
+        h1[j] = a > 0.f ? a : 0.f;
+    }
+    for (int j = 0; j < H; ++j) {
+        float a = P[OFF_B2 + j];
+        for (int i = 0; i < H; ++i) a = std::fmaf(P[OFF_W2 + j * H + i], h1[i], a);
+        h2[j] = a > 0.f ? a : 0.f;
+    }
+    int bi = 0;
+    float bv = 0.f;
+    for (int o = 0; o < OUT; ++o) {
+        float a = P[OFF_B3 + o];
+        for (int i = 0; i < H; ++i) a = std::fmaf(P[OFF_W3 + o * H + i], h2[i], a);
+        if (o == 0 || a < bv) { bv = a; bi = o; }
+    }
+    return bi;
 }
 
 } // namespace blis_amd

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace blis_amd {
 
 constexpr int GEMM_NSCHED = 3;                  // schedules per tile (select.hip)
@@ -13,6 +15,20 @@ constexpr int MLP_IN = 16;
 constexpr int MLP_H = 64;
 constexpr int MLP_OUT = GEMM_NCFG;
 constexpr int MLP_NPARAM = MLP_H * MLP_IN + MLP_H + MLP_H * MLP_H + MLP_H + MLP_OUT * MLP_H + MLP_OUT;
+
+// Host copy of a network for one shape at a time (the selector's product
+// path: no device memory, no launch, no wait). Same arithmetic, in the same
+// order, as the device forward pass, so both give the same scores.
+struct MlpHost {
+    std::vector<float> p;      // parameters (layout of MlpNet)
+    std::vector<float> musd;   // mu[IN], sd[IN]
+    bool ok() const { return !p.empty(); }
+    // argmin over the scores of one raw feature row (first index on ties).
+    int argmin(const float* feats) const;
+};
+
+// Read a weights file written by MlpNet::save.
+bool mlp_read(const char* path, std::vector<float>& params, std::vector<float>& musd);
 
 // Device-resident network: parameters, Adam moments, gradient and the
 // feature normalisation (mu, sd), plus batch scratch buffers.

@@ -10,6 +10,10 @@
 //               features (one inference per new shape, then cached);
 //   exhaustive  time every configuration on scratch operands of the same
 //               shape and keep the fastest (cached).
+// The learned selector is the default (weights shipped beside the library);
+// it runs on the host, a few microseconds per new shape. Single calls with
+// m, n, k <= 512 take the small-problem path before any selector but the
+// exhaustive one.
 // Every configuration computes bit-identical results (same MFMA k order, one
 // fixed epilogue), so the selector changes speed only.
 //
@@ -73,8 +77,7 @@ struct KeyHash {
 struct SelectorState {
     bool initialised = false;
     int mode = SELECT_HEURISTIC;
-    MlpNet* net = nullptr;
-    bool own_net = false;
+    MlpHost net;   // the learned selector's network (host copy)
     std::unordered_map<Key, int, KeyHash> cache;
     bool warmed[2] = {false, false};
 };
@@ -135,22 +138,29 @@ void gemm_features(Dt dt, int64_t m, int64_t n, int64_t k, bool a_k, bool b_k, b
     f[15] = lg((double)std::min(m, std::min(n, k)));
 }
 
+// Default: the learned selector with the weights shipped beside the
+// library (or BLIS_AMD_SELECTOR_WEIGHTS). BLIS_AMD_SELECTOR=heuristic or
+// exhaustive chooses another mode; an explicit "mlp" whose weights cannot be
+// read is an error, while missing default weights leave the heuristic in
+// place with one warning.
 static void selector_init() {
     SelectorState& s = SS();
     if (s.initialised) return;
     s.initialised = true;
     const char* mode = std::getenv("BLIS_AMD_SELECTOR");
-    if (mode == nullptr) return;
+    const char* w = std::getenv("BLIS_AMD_SELECTOR_WEIGHTS");
+    const std::string path = w ? std::string(w) : default_weights_path();
+    if (mode == nullptr) {
+        if (mlp_read(path.c_str(), s.net.p, s.net.musd)) s.mode = SELECT_LEARNED;
+        else std::fprintf(stderr, "libblis_amd: selector weights '%s' not found; using the size "
+                                  "heuristic\n", path.c_str());
+        return;
+    }
     if (!std::strcmp(mode, "mlp") || !std::strcmp(mode, "learned")) {
-        const char* w = std::getenv("BLIS_AMD_SELECTOR_WEIGHTS");
-        const std::string path = w ? std::string(w) : default_weights_path();
-        MlpNet* net = new MlpNet();
-        if (!net->load(path.c_str())) {
+        if (!mlp_read(path.c_str(), s.net.p, s.net.musd)) {
             std::fprintf(stderr, "libblis_amd: cannot load selector weights '%s'\n", path.c_str());
             std::abort();
         }
-        s.net = net;
-        s.own_net = true;
         s.mode = SELECT_LEARNED;
     } else if (!std::strcmp(mode, "exhaustive")) {
         s.mode = SELECT_EXHAUSTIVE;
@@ -262,9 +272,21 @@ int selector_pick(Dt dt, int64_t m, int64_t n, int64_t k, bool a_k, bool b_k, bo
     if (it != s.cache.end()) return it->second;
     int cfg;
     if (s.mode == SELECT_LEARNED) {
-        float f[MLP_IN];
-        gemm_features(dt, m, n, k, a_k, b_k, va, vb, vc, f);
-        s.net->predict(f, 1, nullptr, &cfg);
+        // FP64 products of at least 1024 output tiles of 128 x 128 whose
+        // operands all allow 16-byte access keep the heuristic's large
+        // tile: the band its rule was tuned on by exhaustive search, where
+        // it measured within 1-2% of the best configuration on shapes with
+        // dimensions divisible by 64 (and 88% of the FP64 peak at 8192^3),
+        // while a network's pick there cost 5% at 8192^3. (The FP32 rule
+        // is no better than the network in that band.)
+        const int64_t t128 = ((m + 127) / 128) * ((n + 127) / 128);
+        if (dt == Dt::F64 && t128 >= 1024 && va && vb && vc) {
+            cfg = heuristic_cfg(dt, m, n, k);
+        } else {
+            float f[MLP_IN];
+            gemm_features(dt, m, n, k, a_k, b_k, va, vb, vc, f);
+            cfg = s.net.argmin(f);
+        }
     } else {
         // Scratch operands with the leading dimensions the caller's alignment
         // implies (aligned: rows rounded to 16 bytes; unaligned: rows + 1).
@@ -336,29 +358,17 @@ __attribute__((visibility("default"))) void bli_amd_gemm_set_selector(int mode, 
     if (mode < SELECT_HEURISTIC || mode > SELECT_EXHAUSTIVE)
         abort_with(BLIS_INVALID_UKR_ID, "bli_amd_gemm_set_selector: unknown mode");
     SelectorState& s = SS();
-    auto release = [&s]() {
-        if (s.own_net) delete s.net;
-        s.net = nullptr;
-        s.own_net = false;
-    };
     if (mode == SELECT_LEARNED) {
+        MlpHost h;
         if (net != nullptr) {
-            // A caller's network: borrowed, never freed here.
-            release();
-            s.net = static_cast<MlpNet*>(net);
-        } else if (s.net == nullptr || !s.own_net) {
-            release();
-            MlpNet* n = new MlpNet();
-            const std::string path = default_weights_path();
-            if (!n->load(path.c_str())) {
-                delete n;
-                abort_with(BLIS_EXPECTED_NONNULL_OBJECT_BUFFER, "selector weights not found");
-            }
-            s.net = n;
-            s.own_net = true;
+            // A caller's network: its current parameters are copied.
+            h.p.resize(MLP_NPARAM);
+            h.musd.resize(2 * MLP_IN);
+            static_cast<MlpNet*>(net)->get_params(h.p.data(), h.musd.data());
+        } else if (!mlp_read(default_weights_path().c_str(), h.p, h.musd)) {
+            abort_with(BLIS_EXPECTED_NONNULL_OBJECT_BUFFER, "selector weights not found");
         }
-    } else if (!s.own_net) {
-        s.net = nullptr;   // drop a borrowed network
+        s.net = std::move(h);
     }
     s.mode = mode;
     s.cache.clear();

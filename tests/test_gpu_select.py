@@ -9,7 +9,10 @@ Bars:
     lowers the loss;
   * every configuration gives bit-identical GEMM output;
   * the learned and exhaustive selectors give output bit-identical to the
-    heuristic's.
+    heuristic's;
+  * the learned selector is the default mode, and its host-side inference
+    picks what the HIP inference kernels pick (with the FP64 large-tile
+    band kept on the heuristic's tile).
 """
 from __future__ import annotations
 
@@ -178,6 +181,7 @@ def test_selector_modes_bit_identical_to_heuristic(blis, L, mode, tmp_path):
         # a freshly initialised network (the shipped weights are tested
         # separately): any prediction is a valid configuration
         net = L.bli_amd_mlp_create(3)
+    prev = L.bli_amd_gemm_get_selector()
     try:
         L.bli_amd_gemm_set_selector(mode, net)
         cfg = L.bli_amd_gemm_selected_config(1, 0, 0, m, n, k, m, k, m)
@@ -186,7 +190,7 @@ def test_selector_modes_bit_identical_to_heuristic(blis, L, mode, tmp_path):
         _gemm(blis, "d", m, n, k, a, b, c1)
         assert np.array_equal(c1.device_view(), ref)
     finally:
-        L.bli_amd_gemm_set_selector(0, None)
+        L.bli_amd_gemm_set_selector(prev, None)
         if net:
             L.bli_amd_mlp_destroy(net)
 
@@ -226,6 +230,7 @@ def test_learned_mode_with_shipped_weights(blis, L):
     c0 = Mat(m, n, seed=17).to_device()
     _gemm(blis, "d", m, n, k, a, b, c0)
     ref = c0.device_view().copy()
+    prev = L.bli_amd_gemm_get_selector()
     try:
         L.bli_amd_gemm_set_selector(1, None)
         assert L.bli_amd_gemm_get_selector() == 1
@@ -235,7 +240,7 @@ def test_learned_mode_with_shipped_weights(blis, L):
         _gemm(blis, "d", m, n, k, a, b, c1)
         assert np.array_equal(c1.device_view(), ref)
     finally:
-        L.bli_amd_gemm_set_selector(0, None)
+        L.bli_amd_gemm_set_selector(prev, None)
 
 
 def test_exhaustive_mode_general_stride_b(blis, L):
@@ -257,10 +262,46 @@ def test_exhaustive_mode_general_stride_b(blis, L):
                        C.byref(be), c.ptr, c.rs, c.cs)
     run(c0)
     ref = c0.device_view().copy()
+    prev = L.bli_amd_gemm_get_selector()
     try:
         L.bli_amd_gemm_set_selector(2, None)
         c1 = Mat(m, n, seed=27).to_device()
         run(c1)
         assert np.array_equal(c1.device_view(), ref)
     finally:
-        L.bli_amd_gemm_set_selector(0, None)
+        L.bli_amd_gemm_set_selector(prev, None)
+
+
+def test_learned_selector_is_default_and_host_inference_matches(L):
+    """The default mode is the learned selector (BLIS_AMD_SELECTOR unset).
+    Its host-side inference (bli_amd_gemm_selected_config) picks, on 400
+    held-out shapes of both tuning databases, the configuration that the
+    HIP inference kernels pick with the shipped weights, after the FP64
+    large-tile band rule (tune.guarded)."""
+    import os
+    from pathlib import Path
+    from blis_amd import tune
+    say("learned selector default; host vs HIP inference")
+    if os.environ.get("BLIS_AMD_SELECTOR") or os.environ.get("BLIS_AMD_SELECTOR_WEIGHTS"):
+        pytest.skip("selector chosen through the environment")
+    L.bli_amd_gemm_get_selector.restype = C.c_int
+    L.bli_amd_gemm_selected_config.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_int64] * 6
+    L.bli_amd_gemm_selected_config.restype = C.c_int
+    assert L.bli_amd_gemm_get_selector() == 1
+    data = Path(__file__).resolve().parent.parent / "blis_amd" / "data"
+    db = tune.load_db([data / "gemm_tuning_gfx950.npz", data / "gemm_tuning_round_gfx950.npz"], 0.1)
+    rows = np.where(db["held"])[0][::6][:400]
+    sub = {k: v[rows] for k, v in db.items()}
+    feats = np.ascontiguousarray(sub["feats"], dtype=np.float32)
+    net = L.bli_amd_mlp_load(str(data / "selector_gfx950.bin").encode())
+    assert net
+    dev = np.zeros(len(rows), dtype=np.int32)
+    L.bli_amd_mlp_predict(net, fp(feats), len(rows), None, dev.ctypes.data_as(C.POINTER(C.c_int)))
+    L.bli_amd_mlp_destroy(net)
+    want = tune.guarded(sub, dev, tune.heuristic_picks(L, sub))
+    got = np.array([L.bli_amd_gemm_selected_config(int(sub["dbl"][i]), 8 * int(sub["ta"][i]),
+                                                   8 * int(sub["tb"][i]), *map(int, sub["dims"][i]),
+                                                   int(sub["lda"][i]), int(sub["ldb"][i]),
+                                                   int(sub["ldc"][i]))
+                    for i in range(len(rows))])
+    assert np.array_equal(got, want), np.where(got != want)[0][:10]
