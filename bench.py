@@ -137,6 +137,64 @@ def cpu_baseline():
                       f"the ic loop) on a {s}^3 NN sample, best of 2, {threads} threads"}
 
 
+def cpu_port_extras():
+    """The oracle's TRSM, batch (a loop of its GEMM) and level-1/2 routines
+    timed on one host core, on bounded samples of the bench workloads."""
+    import numpy as np
+    from oracle import pyoracle
+    out = {}
+
+    def best_of(fn, reps=3):
+        b = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            b = min(b, time.perf_counter() - t0)
+        return b
+    rng = np.random.default_rng(11)
+    # DTRSM L/L/N/N, the bench's matrices at m = 1024, n = 256.
+    m, n = 1024, 256
+    a = np.asfortranarray(np.tril(rng.uniform(-1, 1, (m, m))) + 4096.0 * np.eye(m))
+    b0 = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    t = best_of(lambda: pyoracle.trsm(0, 0xC0, "n", 0, 1.0, a, b0.copy(order="F")))
+    out["dtrsm_llnn_1024x256"] = {"ms": round(t * 1e3, 3), "GFLOP/s": round(m * m * n / t / 1e9, 3),
+                                  "cores": 1}
+    # Batch: the first 300 problems of the 10,000-problem workload, one GEMM
+    # call each (the reference's batch loop), extrapolated by flop count.
+    rng2 = np.random.default_rng(42)
+    dims = np.rint(np.exp(rng2.uniform(np.log(8), np.log(512), (10000, 3)))).astype(np.int64)
+    sample = dims[:300]
+    mats = [(np.asfortranarray(rng.uniform(-1, 1, (mm, kk))), np.asfortranarray(rng.uniform(-1, 1, (kk, nn))),
+             np.asfortranarray(rng.uniform(-1, 1, (mm, nn)))) for mm, nn, kk in sample]
+
+    def run_batch():
+        for (x, y, z) in mats:
+            pyoracle.gemm("n", "n", 1.0, x, y, 0.5, z)
+    t = best_of(run_batch, 2)
+    fl_s = float(np.sum(2.0 * sample[:, 0] * sample[:, 1] * sample[:, 2]))
+    fl_all = float(np.sum(2.0 * dims[:, 0] * dims[:, 1] * dims[:, 2]))
+    out["dgemm_batch_10000"] = {"sample_problems": 300, "sample_ms": round(t * 1e3, 2),
+                                "GFLOP/s": round(fl_s / t / 1e9, 3),
+                                "ms_extrapolated": round(t * fl_all / fl_s * 1e3, 1), "cores": 1}
+    # Level-1/2 on one core.
+    n1 = 1 << 24
+    x = rng.uniform(-1, 1, n1)
+    y = rng.uniform(-1, 1, n1)
+    t = best_of(lambda: pyoracle.axpyv(0.5, x, y))
+    out["daxpyv_2^24"] = {"ms": round(t * 1e3, 3), "GB/s": round(24.0 * n1 / t / 1e9, 3), "cores": 1}
+    t = best_of(lambda: pyoracle.dotv(x, y))
+    out["ddotv_2^24"] = {"ms": round(t * 1e3, 3), "GB/s": round(16.0 * n1 / t / 1e9, 3), "cores": 1}
+    mg = 4096
+    ag = np.asfortranarray(rng.uniform(-1, 1, (mg, mg)))
+    xg = rng.uniform(-1, 1, mg)
+    yg = rng.uniform(-1, 1, mg)
+    for tr in ("n", "t"):
+        t = best_of(lambda: pyoracle.gemv(tr, 1.0, ag, xg, 0.5, yg))
+        out[f"dgemv_{tr}_4096"] = {"ms": round(t * 1e3, 3),
+                                   "GB/s": round(8.0 * (mg * mg + 3 * mg) / t / 1e9, 3), "cores": 1}
+    return out
+
+
 def trsm_ms(bench: Bench, m=4096, n=1024):
     """DTRSM L/L/N/N: A lower with diagonal 4096 + U[0,1] and off-diagonal
     U[-1,1], B U[-1,1], alpha = 1. B is restored before every call; the
@@ -520,9 +578,41 @@ def batch_bench(bench, ch, key, log, count=10000, seed=42, replays=None):
         L.bli_amd_gemm_batch_plan_set_mode(plan, mode)
         modes[mode] = round(bench.kernel_ms(lambda: L.bli_amd_gemm_batch_plan_execute(plan, 1), 10), 4)
     tuned = L.bli_amd_gemm_batch_plan_autotune(plan, 5)
+    single = L.bli_amd_gemm_batch_plan_get_single_dispatch(plan)
     ms = bench.kernel_ms(lambda: L.bli_amd_gemm_batch_plan_execute(plan, 1), 20)
     L.bli_amd_gemm_batch_plan_destroy(plan)
-    log(f"  {ch}gemm_batch modes (ms): {modes}; autotuned mode {tuned}")
+    log(f"  {ch}gemm_batch modes (ms): {modes}; autotuned mode {tuned}, single dispatch {single}")
+    # The drop-in call (bli_?gemm_batch, synchronous): the first call builds
+    # and caches the plan, the second autotunes it, later calls replay it.
+    drop = getattr(L, f"bli_{ch}gemm_batch")
+    walls = []
+    for _ in range(12):
+        t0 = time.perf_counter()
+        drop(*cargs)
+        walls.append((time.perf_counter() - t0) * 1e3)
+    dropin = {"first_call_ms": round(walls[0], 3), "second_call_ms": round(walls[1], 3),
+              "cached_call_ms": round(float(np.median(walls[2:])), 4)}
+    # The same problems as 10,000 plain bli_?gemm calls, enqueued without
+    # waiting between them (device time between events around the loop).
+    single_gemm = getattr(L, f"bli_{ch}gemm")
+    pa = [A.data_ptr() + int(o) * es for o in oa[:-1]]
+    pb = [B.data_ptr() + int(o) * es for o in ob[:-1]]
+    pc = [Cm.data_ptr() + int(o) * es for o in oc[:-1]]
+    al_c = [T(float(x)) for x in alpha]
+    be_c = [T(float(x)) for x in beta]
+    calls = [(int(ta[i]) * 8, int(tb[i]) * 8, int(dims[i, 0]), int(dims[i, 1]), int(dims[i, 2]),
+              C.byref(al_c[i]), pa[i], 1, int(lda[i]), pb[i], 1, int(ldb[i]), C.byref(be_c[i]),
+              pc[i], 1, int(ldc[i])) for i in range(count)]
+
+    def loop():
+        for a_ in calls:
+            single_gemm(*a_)
+    t0 = time.perf_counter()
+    per_ms = bench.kernel_ms(loop, 1)
+    per_wall = (time.perf_counter() - t0) * 1e3 / 2   # kernel_ms runs the loop twice
+    log(f"  {ch}gemm_batch drop-in call: first {dropin['first_call_ms']:.2f} ms, second "
+        f"{dropin['second_call_ms']:.2f} ms, cached {dropin['cached_call_ms']:.3f} ms; "
+        f"10,000 plain calls: device {per_ms:.3f} ms, host loop {per_wall:.1f} ms")
     m, n, k = dims[:, 0].astype(float), dims[:, 1].astype(float), dims[:, 2].astype(float)
     flops = 2 * m * n * k
     byts = es * (m * k + k * n + 2 * m * n)
@@ -533,7 +623,18 @@ def batch_bench(bench, ch, key, log, count=10000, seed=42, replays=None):
                                      "roofline_ms": round(roof_ms, 4),
                                      "frac": round(roof_ms / ms, 4),
                                      "bound": "per-problem max(mfma, hbm)",
-                                     "mode_ms": modes, "autotuned_mode": tuned}}
+                                     "mode_ms": modes, "autotuned_mode": tuned,
+                                     "single_dispatch": single},
+           f"{ch}gemm_batch_10000_dropin": {**dropin,
+                                            "frac": round(roof_ms / dropin["cached_call_ms"], 4),
+                                            "note": "synchronous bli_?gemm_batch call, host "
+                                                    "time (argument checks, pointer "
+                                                    "classification, plan lookup, launch, wait)"},
+           f"{ch}gemm_batch_10000_plain_calls": {"ms": round(per_ms, 4),
+                                                 "host_loop_ms": round(per_wall, 2),
+                                                 "frac": round(roof_ms / per_ms, 4),
+                                                 "note": "10,000 bli_?gemm calls enqueued "
+                                                         "back to back, device time"}}
     log(f"  {ch}gemm_batch_10000{'':17s} {ms:9.4f} ms  {tf:9.3f} TFLOP/s  "
         f"({100 * roof_ms / ms:5.1f}% of per-problem roofline {roof_ms:.3f} ms)")
     del A, B, Cm
@@ -686,6 +787,9 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline()
     if cpu:
         log(f"  cpu baseline: {cpu['value']} TFLOP/s on {cpu['cores']} threads")
+        if not args.no_extras:
+            extra["cpu_port"] = cpu_port_extras()
+            log(f"  cpu port (1 core): {json.dumps(extra['cpu_port'])}")
 
     line = {
         "metric": "DGEMM/SGEMM TFLOP/s (% MFMA peak): native vs split-precision at matched error",

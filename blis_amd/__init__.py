@@ -136,6 +136,8 @@ def _sig(lib):
     s("bli_amd_gemm_batch_plan_autotune", C.c_int, P, C.c_int)
     s("bli_amd_gemm_batch_plan_set_mode", None, P, C.c_int)
     s("bli_amd_gemm_batch_plan_get_mode", C.c_int, P)
+    s("bli_amd_gemm_batch_plan_set_single_dispatch", None, P, C.c_int)
+    s("bli_amd_gemm_batch_plan_get_single_dispatch", C.c_int, P)
     s("bli_amd_set_stream", None, P, C.c_int)
     s("bli_amd_get_stream", P)
     s("bli_amd_synchronize", None)

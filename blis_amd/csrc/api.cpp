@@ -147,6 +147,7 @@ void gemm_batch_core(const trans_t* transa, const trans_t* transb,
     }
     std::lock_guard<std::mutex> lock(api_mutex());
     probs.reserve((size_t)total);
+    DeviceRanges ranges;
     dim_t idx = 0;
     for (dim_t g = 0; g < group_count; ++g) {
         const bool ta = does_trans(transa[g]), tb = does_trans(transb[g]);
@@ -159,13 +160,13 @@ void gemm_batch_core(const trans_t* transa, const trans_t* transb,
             if (!touch_ab) p.k = 0;   // C := beta*C without reading A or B
             stages.emplace_back();
             Staged& sa = stages.back();
-            if (touch_ab) stage(sa, a[idx], dt, ta ? k[g] : m[g], ta ? m[g] : k[g], rs_a[g], cs_a[g], false);
+            if (touch_ab) stage(sa, a[idx], dt, ta ? k[g] : m[g], ta ? m[g] : k[g], rs_a[g], cs_a[g], false, &ranges);
             stages.emplace_back();
             Staged& sb = stages.back();
-            if (touch_ab) stage(sb, b[idx], dt, tb ? n[g] : k[g], tb ? k[g] : n[g], rs_b[g], cs_b[g], false);
+            if (touch_ab) stage(sb, b[idx], dt, tb ? n[g] : k[g], tb ? k[g] : n[g], rs_b[g], cs_b[g], false, &ranges);
             stages.emplace_back();
             Staged& sc = stages.back();
-            stage(sc, c[idx], dt, m[g], n[g], rs_c[g], cs_c[g], true);
+            stage(sc, c[idx], dt, m[g], n[g], rs_c[g], cs_c[g], true, &ranges);
             p.a = { sa.dev, ta ? cs_a[g] : rs_a[g], ta ? rs_a[g] : cs_a[g] };
             p.b = { sb.dev, tb ? cs_b[g] : rs_b[g], tb ? rs_b[g] : cs_b[g] };
             p.c = { sc.dev, rs_c[g], cs_c[g] };

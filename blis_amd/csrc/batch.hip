@@ -19,6 +19,7 @@
 #include "gemm_tile.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <numeric>
 
 namespace blis_amd {
@@ -147,6 +148,101 @@ __attribute__((amdgpu_waves_per_eu(mode_waves(MODE)))) void gemm_batch_kernel(co
     }
 }
 
+// Variant v = 4*small + 2*a_k + b_k.
+constexpr int NVARIANT = 8;
+__host__ __device__ inline int variant_of(int32_t flags) {
+    return ((flags & F_SMALL) ? 4 : 0) + ((flags & F_A_K) ? 2 : 0) + ((flags & F_B_K) ? 1 : 0);
+}
+
+// The whole batch in one dispatch (every mode but the wave mode). Work
+// items of all variants sit in one list per XCD, longest first; workgroup b
+// runs on XCD b % 8 and takes the next item of that list from a device
+// counter (ctr[x]), so the variants share the machine and no workgroup
+// idles while its XCD has work left. The index two items ahead is fetched,
+// and the next item's descriptor loaded, while the current tile runs. The
+// last workgroup to finish sets the counters back to zero for the next
+// launch (ctr[NXCD] counts finished workgroups), so a plan's launches on
+// one stream need no reset in between; two launches of one plan must not
+// overlap.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(mode_waves(MODE)))) void gemm_batch_dyn_kernel(
+    const BatchDesc* __restrict__ descs, const WorkItem* __restrict__ items, XcdSegs seg,
+    int* __restrict__ ctr) {
+    using Big = typename std::conditional<(MODE >= 7), typename BatchTiles<T>::BigHalfK,
+                                          typename BatchTiles<T>::Big>::type;
+    using Small = typename BatchTiles<T>::Small;
+    static_assert(Big::NT == 256 && Small::NT == 256, "one workgroup shape for every variant");
+    constexpr int SMEM = 2 * (Big::STAGE > Small::STAGE ? Big::STAGE : Small::STAGE);
+    __shared__ T smem[SMEM];
+    __shared__ int sink[mode_c_pf<T>(MODE) ? 256 : 1];
+    __shared__ int shared_idx[2];
+    const int x = blockIdx.x % NXCD;
+    const int beg = seg.off[x], end = seg.off[x + 1];
+    if (threadIdx.x == 0) {
+        shared_idx[0] = beg + atomicAdd(&ctr[x], 1);
+        shared_idx[1] = beg + atomicAdd(&ctr[x], 1);
+    }
+    __syncthreads();
+    // Uniform across the workgroup: keep the indices (and so the item and
+    // descriptor loads) scalar.
+    int cur = __builtin_amdgcn_readfirstlane(shared_idx[0]);
+    int nxt = __builtin_amdgcn_readfirstlane(shared_idx[1]);
+    WorkItem w{};
+    BatchDesc d{};
+    if (cur < end) {
+        w = items[cur];
+        d = descs[w.prob];
+    }
+    while (cur < end) {
+        int nn = end;
+        if (threadIdx.x == 0 && nxt < end) nn = beg + atomicAdd(&ctr[x], 1);
+        WorkItem wn{};
+        BatchDesc dn{};
+        if (nxt < end) {
+            wn = items[nxt];
+            dn = descs[wn.prob];
+        }
+        TileProblem<T> p;
+        p.a = (const T*)d.a; p.b = (const T*)d.b; p.c = (T*)d.c;
+        p.m = d.m; p.n = d.n; p.k = d.k;
+        p.lda = d.lda; p.ldb = d.ldb; p.c_si = d.c_si; p.c_sj = d.c_sj;
+        p.alpha = (T)d.alpha; p.beta = (T)d.beta;
+        p.vec_a = (d.flags & F_VEC_A) != 0;
+        p.vec_b = (d.flags & F_VEC_B) != 0;
+        p.vec_c = (d.flags & F_VEC_C) != 0;
+        const int64_t tm = w.tile % d.tiles_m, tn = w.tile / d.tiles_m;
+        constexpr int PF = mode_pf(MODE);
+        constexpr bool EG = mode_edge(MODE), LE = mode_lds_epi<T>(MODE), CP = mode_c_pf<T>(MODE);
+        switch (variant_of(d.flags)) {
+        case 0: gemm_tile<T, Big, false, false, PF, EG, LE, CP>(p, tm, tn, smem, sink); break;
+        case 1: gemm_tile<T, Big, false, true, PF, EG, LE, CP>(p, tm, tn, smem, sink); break;
+        case 2: gemm_tile<T, Big, true, false, PF, EG, LE, CP>(p, tm, tn, smem, sink); break;
+        case 3: gemm_tile<T, Big, true, true, PF, EG, LE, CP>(p, tm, tn, smem, sink); break;
+        case 4: gemm_tile<T, Small, false, false, PF, EG, LE, CP>(p, tm, tn, smem, sink); break;
+        case 5: gemm_tile<T, Small, false, true, PF, EG, LE, CP>(p, tm, tn, smem, sink); break;
+        case 6: gemm_tile<T, Small, true, false, PF, EG, LE, CP>(p, tm, tn, smem, sink); break;
+        default: gemm_tile<T, Small, true, true, PF, EG, LE, CP>(p, tm, tn, smem, sink); break;
+        }
+        if (threadIdx.x == 0) shared_idx[0] = nn;
+        __syncthreads();   // smem free for the next tile; nn visible
+        cur = nxt;
+        w = wn;
+        d = dn;
+        nxt = __builtin_amdgcn_readfirstlane(shared_idx[0]);
+        __syncthreads();   // everyone has read shared_idx before it is rewritten
+    }
+    // All of this workgroup's counter fetches are done. The last workgroup
+    // to get here resets the counters (vector atomics).
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&ctr[NXCD], 1) == (int)gridDim.x - 1) {
+            for (int i = 0; i < NXCD; ++i) atomicExch(&ctr[i], 0);
+            atomicExch(&ctr[NXCD], 0);
+        }
+    }
+}
+
 // Wave-tile kernel (mode 4): one wave computes one 32 x 32 tile of C as 2 x 2
 // MFMA blocks of 16 x 16, loading its MFMA operand fragments straight from
 // global memory (no LDS, no barriers), one k-step ahead. For the small
@@ -240,12 +336,6 @@ __global__ __launch_bounds__(256) void gemm_batch_wave_kernel(const BatchDesc* _
     }
 }
 
-// Variant v = 4*small + 2*a_k + b_k.
-constexpr int NVARIANT = 8;
-inline int variant_of(int32_t flags) {
-    return ((flags & F_SMALL) ? 4 : 0) + ((flags & F_A_K) ? 2 : 0) + ((flags & F_B_K) ? 1 : 0);
-}
-
 template <typename T, int MODE>
 inline const void* variant_fn(int v) {
     using Big = typename std::conditional<(MODE >= 7), typename BatchTiles<T>::BigHalfK,
@@ -290,6 +380,22 @@ inline void variant_info(int mode, int v, const void*& fn, int& threads) {
     case 8: fn = variant_fn<T, 8>(v); return;
     }
     abort_with(BLIS_INVALID_UKR_ID, "batch: mode");
+}
+
+template <typename T>
+inline const void* dyn_fn(int mode) {
+    switch (mode) {
+    case 0: return (const void*)&gemm_batch_dyn_kernel<T, 0>;
+    case 1: return (const void*)&gemm_batch_dyn_kernel<T, 1>;
+    case 2: return (const void*)&gemm_batch_dyn_kernel<T, 2>;
+    case 3: return (const void*)&gemm_batch_dyn_kernel<T, 3>;
+    case 5: return (const void*)&gemm_batch_dyn_kernel<T, 5>;
+    case 6: return (const void*)&gemm_batch_dyn_kernel<T, 6>;
+    case 7: return (const void*)&gemm_batch_dyn_kernel<T, 7>;
+    case 8: return (const void*)&gemm_batch_dyn_kernel<T, 8>;
+    }
+    abort_with(BLIS_INVALID_UKR_ID, "batch: single-dispatch mode");
+    return nullptr;
 }
 
 // C of every problem to / from a contiguous scratch copy (the autotuner
@@ -339,6 +445,18 @@ struct BatchPlan {
     XcdSegs wseg[4] = {};             // per layout variant (2*a_k + b_k)
     int mode = 0;                     // kernel mode used by execute
     int vmode[NVARIANT] = {};         // per-variant kernel mode (mode != WAVE_MODE)
+    // Single dispatch (gemm_batch_dyn_kernel): every variant's items in one
+    // list per XCD, the workgroup count per mode, the per-XCD counters. Off
+    // until the autotuner picks it: on the 10,000-problem batch (dims
+    // log-uniform in [8, 512]) the kernel holding all eight tile paths
+    // needs more registers (FP64 mode 8: 320 bytes per lane of spills
+    // against 20-64 for the per-variant kernels) and measured slower
+    // (d 1.83 ms at best against 1.74 ms, s 1.24 against 1.03 ms).
+    bool dyn = false;
+    WorkItem* d_ditems = nullptr;
+    XcdSegs dseg = {};
+    int dgrid[NMODE] = {};
+    int* d_ctr = nullptr;
     int ndesc = 0;
     int64_t c_elems = 0;              // sum of m*n (autotuner scratch)
     std::vector<int64_t> c_offs;
@@ -352,6 +470,8 @@ struct BatchPlan {
         if (d_descs) (void)hipFree(d_descs);
         if (d_items) (void)hipFree(d_items);
         if (d_witems) (void)hipFree(d_witems);
+        if (d_ditems) (void)hipFree(d_ditems);
+        if (d_ctr) (void)hipFree(d_ctr);
     }
 };
 
@@ -516,7 +636,71 @@ static BatchPlan* plan_build(Dt dt, const std::vector<BatchProblem>& probs, hipS
                                      wsorted.size() * sizeof(WorkItem), hipMemcpyHostToDevice, st));
         }
     }
+    // Single-dispatch lists: the items of every variant, placed on the XCDs
+    // by problem (greedy, largest first, on an estimate of each tile's time:
+    // k slabs of 16 plus two for the epilogue, a 32 x 32 tile at 0.4 of a
+    // 64 x 64 one), then longest tile first within each XCD (ties keep the
+    // submission order, so a problem's tiles stay together).
+    std::vector<int32_t> dx(descs.size(), 0);
+    {
+        auto tile_cost = [&](const BatchDesc& d) {
+            return ((d.flags & F_SMALL) ? 0.4 : 1.0) * (double)((std::max<int32_t>(d.k, 0) + 15) / 16 + 2);
+        };
+        std::vector<double> pwork(descs.size(), 0.0);
+        for (size_t i = 0; i < ni; ++i) pwork[items[i].prob] += tile_cost(descs[items[i].prob]);
+        std::vector<int32_t> porder(descs.size());
+        std::iota(porder.begin(), porder.end(), 0);
+        std::stable_sort(porder.begin(), porder.end(),
+                         [&](int32_t x, int32_t y) { return pwork[x] > pwork[y]; });
+        double load[NXCD] = {};
+        for (int32_t pi : porder) {
+            int best = 0;
+            for (int x = 1; x < NXCD; ++x)
+                if (load[x] < load[best]) best = x;
+            dx[pi] = best;
+            load[best] += pwork[pi];
+        }
+        std::vector<int32_t> dord(ni);
+        std::iota(dord.begin(), dord.end(), 0);
+        std::stable_sort(dord.begin(), dord.end(), [&](int32_t x, int32_t y) {
+            const int32_t px = dx[items[x].prob], py = dx[items[y].prob];
+            if (px != py) return px < py;
+            return tile_cost(descs[items[x].prob]) > tile_cost(descs[items[y].prob]);
+        });
+        int dcnt[NXCD] = {};
+        for (size_t i = 0; i < ni; ++i) dcnt[dx[items[i].prob]]++;
+        int doff = 0;
+        for (int x = 0; x <= NXCD; ++x) {
+            plan->dseg.off[x] = doff;
+            if (x < NXCD) doff += dcnt[x];
+        }
+        if (ni > 0) {
+            std::vector<WorkItem> dsorted(ni);
+            for (size_t i = 0; i < ni; ++i) dsorted[i] = items[dord[i]];
+            HIP_CHECK(hipMalloc(&plan->d_ditems, ni * sizeof(WorkItem)));
+            HIP_CHECK(hipMemcpyAsync(plan->d_ditems, dsorted.data(), ni * sizeof(WorkItem),
+                                     hipMemcpyHostToDevice, st));
+        }
+        static const int zeros[NXCD + 1] = {};
+        HIP_CHECK(hipMalloc(&plan->d_ctr, sizeof(zeros)));
+        HIP_CHECK(hipMemcpyAsync(plan->d_ctr, zeros, sizeof(zeros), hipMemcpyHostToDevice, st));
+    }
     HIP_CHECK(hipStreamSynchronize(st));
+    // Single dispatch: one resident wave of workgroups, a multiple of the
+    // XCD count, and no more per XCD than its longest list.
+    {
+        int longest = 0;
+        for (int x = 0; x < NXCD; ++x)
+            longest = std::max(longest, plan->dseg.off[x + 1] - plan->dseg.off[x]);
+        for (int mode = 0; mode < NMODE; ++mode) {
+            if (mode == WAVE_MODE || longest == 0) continue;
+            const void* fn = dt == Dt::F64 ? dyn_fn<double>(mode) : dyn_fn<float>(mode);
+            int per_cu = 0;
+            HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0));
+            per_cu = std::max(1, per_cu);
+            plan->dgrid[mode] = std::max(1, std::min(longest, cu_count() * per_cu / NXCD)) * NXCD;
+        }
+    }
     // One resident wave of workgroups per variant and mode (persistent
     // kernels), a multiple of the XCD count, and no more per XCD than its
     // longest list.
@@ -565,9 +749,24 @@ static void launch_variant(const BatchPlan* plan, int mode, int v, hipStream_t s
     HIP_CHECK(hipLaunchKernel(fn, dim3(plan->grid[mode][v]), dim3(threads), args, 0, st));
 }
 
+// The single dispatch in kernel mode `mode` (not the wave mode).
+static void launch_dyn(const BatchPlan* plan, int mode, hipStream_t st) {
+    if (plan->dgrid[mode] == 0) return;
+    const void* fn = plan->dt == Dt::F64 ? dyn_fn<double>(mode) : dyn_fn<float>(mode);
+    const BatchDesc* descs = plan->d_descs;
+    const WorkItem* items = plan->d_ditems;
+    XcdSegs seg = plan->dseg;
+    int* ctr = plan->d_ctr;
+    void* args[] = {(void*)&descs, (void*)&items, (void*)&seg, (void*)&ctr};
+    HIP_CHECK(hipLaunchKernel(fn, dim3(plan->dgrid[mode]), dim3(256), args, 0, st));
+}
+
 static void plan_launch(const BatchPlan* plan, hipStream_t st, bool with_fallback = true) {
-    for (int v = 0; v < NVARIANT; ++v)
-        launch_variant(plan, plan->mode == WAVE_MODE ? WAVE_MODE : plan->vmode[v], v, st);
+    if (plan->dyn && plan->mode != WAVE_MODE)
+        launch_dyn(plan, plan->mode, st);
+    else
+        for (int v = 0; v < NVARIANT; ++v)
+            launch_variant(plan, plan->mode == WAVE_MODE ? WAVE_MODE : plan->vmode[v], v, st);
     if (!with_fallback) return;
     for (const BatchProblem& q : plan->fallback) {
         GemmDesc d;
@@ -577,11 +776,109 @@ static void plan_launch(const BatchPlan* plan, hipStream_t st, bool with_fallbac
     }
 }
 
+namespace detail_batch {
+
+bool same_view(const View& x, const View& y) {
+    return x.ptr == y.ptr && x.rs == y.rs && x.cs == y.cs;
+}
+bool same_problem(const BatchProblem& x, const BatchProblem& y) {
+    return x.m == y.m && x.n == y.n && x.k == y.k && same_view(x.a, y.a) &&
+           same_view(x.b, y.b) && same_view(x.c, y.c) && x.alpha == y.alpha && x.beta == y.beta;
+}
+uint64_t batch_hash(Dt dt, const std::vector<BatchProblem>& probs) {
+    uint64_t h = 1469598103934665603ull ^ (uint64_t)dt;
+    auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+    for (const BatchProblem& q : probs) {
+        mix((uint64_t)q.m); mix((uint64_t)q.n); mix((uint64_t)q.k);
+        mix((uint64_t)(uintptr_t)q.a.ptr); mix((uint64_t)q.a.rs); mix((uint64_t)q.a.cs);
+        mix((uint64_t)(uintptr_t)q.b.ptr); mix((uint64_t)q.b.rs); mix((uint64_t)q.b.cs);
+        mix((uint64_t)(uintptr_t)q.c.ptr); mix((uint64_t)q.c.rs); mix((uint64_t)q.c.cs);
+        uint64_t u;
+        std::memcpy(&u, &q.alpha, 8); mix(u);
+        std::memcpy(&u, &q.beta, 8); mix(u);
+    }
+    return h;
+}
+
+// Plans of recent drop-in calls. A plan is a pure function of the call's
+// arguments (dimensions, pointers, strides, scalars), so a call whose
+// arguments equal an earlier one's reuses its device descriptors, work
+// lists and captured graph instead of rebuilding them. A plan reused for
+// the second time is autotuned first (all kernel modes timed on the plan
+// itself, C saved and restored), unless BLIS_AMD_BATCH_AUTOTUNE=0.
+struct CachedPlan {
+    Dt dt;
+    uint64_t hash;
+    std::vector<BatchProblem> probs;
+    BatchPlan* plan;
+    int uses;
+    uint64_t last;
+};
+struct PlanCache {
+    static constexpr size_t CAP = 4;
+    std::vector<CachedPlan> e;
+    uint64_t clock = 0;
+};
+// Never destroyed: the plans' device memory and graphs are released with
+// the process, not by static destructors that may run after the HIP
+// runtime has shut down.
+PlanCache& plan_cache() {
+    static PlanCache* pc = new PlanCache();
+    return *pc;
+}
+bool batch_autotune_enabled() {
+    static const bool on = [] {
+        const char* v = std::getenv("BLIS_AMD_BATCH_AUTOTUNE");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
+} // namespace detail_batch
+
+static void plan_tune(BatchPlan* plan, int reps);
+
+// The drop-in batch (bli_?gemm_batch, ?gemm_batch_): launch a cached plan
+// when one matches the call, else build one (and cache it). Returns when the
+// batch is done (the caller synchronises as BLIS semantics require).
 void gemm_batch_run(Dt dt, const std::vector<BatchProblem>& probs, hipStream_t st) {
+    PlanCache& pc = plan_cache();
+    const uint64_t h = batch_hash(dt, probs);
+    ++pc.clock;
+    for (CachedPlan& c : pc.e) {
+        if (c.dt != dt || c.hash != h || c.probs.size() != probs.size()) continue;
+        bool same = true;
+        for (size_t i = 0; i < probs.size() && same; ++i) same = same_problem(c.probs[i], probs[i]);
+        if (!same) continue;
+        c.last = pc.clock;
+        ++c.uses;
+        BatchPlan* plan = c.plan;
+        if (c.uses == 2 && batch_autotune_enabled()) plan_tune(plan, 3);
+        if (plan->fallback.empty()) {
+            if (!plan->exec) {
+                HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                plan_launch(plan, st);
+                HIP_CHECK(hipStreamEndCapture(st, &plan->graph));
+                HIP_CHECK(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+            }
+            HIP_CHECK(hipGraphLaunch(plan->exec, st));
+        } else {
+            plan_launch(plan, st);
+        }
+        HIP_CHECK(hipStreamSynchronize(st));
+        return;
+    }
     BatchPlan* plan = plan_build(dt, probs, st);
     plan_launch(plan, st);
     HIP_CHECK(hipStreamSynchronize(st));
-    delete plan;
+    if (pc.e.size() >= PlanCache::CAP) {
+        size_t old = 0;
+        for (size_t i = 1; i < pc.e.size(); ++i)
+            if (pc.e[i].last < pc.e[old].last) old = i;
+        delete pc.e[old].plan;
+        pc.e.erase(pc.e.begin() + (long)old);
+    }
+    pc.e.push_back(CachedPlan{dt, h, probs, plan, 1, pc.clock});
 }
 
 } // namespace blis_amd
@@ -676,7 +973,17 @@ __attribute__((visibility("default")))
 int bli_amd_gemm_batch_plan_autotune(void* handle, int reps) {
     auto* plan = (BatchPlan*)handle;
     std::lock_guard<std::mutex> lock(api_mutex());
-    if (plan->nitems == 0) return plan->mode;
+    plan_tune(plan, reps);
+    return plan->mode;
+}
+
+} // extern "C"
+
+namespace blis_amd {
+
+// The batch autotuner (the caller holds the API mutex).
+static void plan_tune(BatchPlan* plan, int reps) {
+    if (plan->nitems == 0) return;
     hipStream_t st = stream();
     reps = std::max(1, reps);
     const size_t es = dt_size(plan->dt);
@@ -729,13 +1036,28 @@ int bli_amd_gemm_batch_plan_autotune(void* handle, int reps) {
     const float wave_ms = time_ms([&] {
         for (int v = 0; v < NVARIANT; ++v) launch_variant(plan, WAVE_MODE, v, st);
     });
+    // The single dispatch in every LDS-tile mode.
+    int dbest = default_mode(plan->dt);
+    float dbest_ms = 0.f;
+    for (int mode = 0; mode < NMODE; ++mode) {
+        if (mode == WAVE_MODE || plan->dgrid[mode] == 0) continue;
+        const float ms = time_ms([&] { launch_dyn(plan, mode, st); });
+        if (dbest_ms == 0.f || ms < dbest_ms) { dbest_ms = ms; dbest = mode; }
+    }
     HIP_CHECK(hipEventDestroy(e0));
     HIP_CHECK(hipEventDestroy(e1));
     copy_c(0);
     HIP_CHECK(hipStreamSynchronize(st));
     bool changed = false;
-    if (wave_ms < sum_ms) {
-        changed = plan->mode != WAVE_MODE;
+    const bool was_dyn = plan->dyn;
+    plan->dyn = dbest_ms > 0.f && dbest_ms < sum_ms && dbest_ms < wave_ms;
+    changed = was_dyn != plan->dyn;
+    if (plan->dyn) {
+        changed = changed || plan->mode != dbest;
+        plan->mode = dbest;
+        for (int v = 0; v < NVARIANT; ++v) plan->vmode[v] = dbest;
+    } else if (wave_ms < sum_ms) {
+        changed = changed || plan->mode != WAVE_MODE;
         plan->mode = WAVE_MODE;
         for (int v = 0; v < NVARIANT; ++v) plan->vmode[v] = WAVE_MODE;
     } else {
@@ -743,7 +1065,7 @@ int bli_amd_gemm_batch_plan_autotune(void* handle, int reps) {
         int top = 0;
         for (int v = 1; v < NVARIANT; ++v)
             if (vbest_ms[v] > vbest_ms[top]) top = v;
-        changed = plan->mode != vbest[top];
+        changed = changed || plan->mode != vbest[top];
         plan->mode = vbest[top];
         for (int v = 0; v < NVARIANT; ++v) {
             changed = changed || plan->vmode[v] != vbest[v];
@@ -756,8 +1078,11 @@ int bli_amd_gemm_batch_plan_autotune(void* handle, int reps) {
         plan->exec = nullptr;
         plan->graph = nullptr;
     }
-    return plan->mode;
 }
+
+} // namespace blis_amd
+
+extern "C" {
 
 __attribute__((visibility("default")))
 void bli_amd_gemm_batch_plan_set_mode(void* handle, int mode) {
@@ -774,6 +1099,26 @@ void bli_amd_gemm_batch_plan_set_mode(void* handle, int mode) {
     }
     plan->mode = mode;
     for (int v = 0; v < NVARIANT; ++v) plan->vmode[v] = mode;
+}
+
+// on != 0: run the plan as one dispatch (every mode but the wave mode);
+// 0: one kernel per variant, one after another.
+__attribute__((visibility("default")))
+void bli_amd_gemm_batch_plan_set_single_dispatch(void* handle, int on) {
+    auto* plan = (BatchPlan*)handle;
+    std::lock_guard<std::mutex> lock(api_mutex());
+    if ((on != 0) != plan->dyn && plan->exec) {
+        HIP_CHECK(hipGraphExecDestroy(plan->exec));
+        HIP_CHECK(hipGraphDestroy(plan->graph));
+        plan->exec = nullptr;
+        plan->graph = nullptr;
+    }
+    plan->dyn = on != 0;
+}
+
+__attribute__((visibility("default")))
+int bli_amd_gemm_batch_plan_get_single_dispatch(void* handle) {
+    return ((BatchPlan*)handle)->dyn ? 1 : 0;
 }
 
 __attribute__((visibility("default")))

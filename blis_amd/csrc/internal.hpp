@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "../../include/blis.h"
@@ -90,8 +91,17 @@ struct Staged {
     Staged(const Staged&) = delete;
     Staged& operator=(const Staged&) = delete;
 };
+// Device allocations seen during one call: a pointer inside a known range
+// is device memory without another runtime query (a batch of 10,000
+// problems in a few arenas then asks the runtime a few times, not 30,000).
+// Valid for one call only: memory may be freed and reused between calls.
+struct DeviceRanges {
+    std::vector<std::pair<uintptr_t, uintptr_t>> r;   // [begin, end), sorted by begin
+    bool is_device(const void* p);
+};
 BLIS_AMD_HIDDEN void stage(Staged& s, const void* p, Dt dt, int64_t m, int64_t n,
-                           int64_t rs, int64_t cs, bool is_output);
+                           int64_t rs, int64_t cs, bool is_output,
+                           DeviceRanges* ranges = nullptr);
 BLIS_AMD_HIDDEN void stage_vec(Staged& s, const void* p, Dt dt, int64_t n,
                                int64_t inc, bool is_output);
 // A complex matrix (real precision dt, strides in complex elements).

@@ -13,6 +13,9 @@
 //                                                   FULL error-checking mode.
 #include "internal.hpp"
 
+#include <algorithm>
+#include <iterator>
+
 #include <atomic>
 #include <cstring>
 #include <string>
@@ -152,9 +155,30 @@ static void span_of(int64_t m, int64_t n, int64_t rs, int64_t cs, int64_t& lo, i
     hi = (r > 0 ? r : 0) + (c > 0 ? c : 0);
 }
 
-static void stage_span(Staged& s, const void* p, Dt dt, int64_t lo, int64_t hi, bool is_output) {
+bool DeviceRanges::is_device(const void* p) {
+    if (p == nullptr) return true;
+    const uintptr_t u = (uintptr_t)p;
+    auto it = std::upper_bound(r.begin(), r.end(), u,
+                               [](uintptr_t v, const std::pair<uintptr_t, uintptr_t>& e) {
+                                   return v < e.first;
+                               });
+    if (it != r.begin() && u < std::prev(it)->second) return true;
+    if (!is_device_pointer(p)) return false;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) == hipSuccess && size > 0) {
+        const std::pair<uintptr_t, uintptr_t> e{(uintptr_t)base, (uintptr_t)base + size};
+        r.insert(std::upper_bound(r.begin(), r.end(), e), e);
+    } else {
+        (void)hipGetLastError();
+    }
+    return true;
+}
+
+static void stage_span(Staged& s, const void* p, Dt dt, int64_t lo, int64_t hi, bool is_output,
+                       DeviceRanges* ranges = nullptr) {
     s.dev = const_cast<void*>(p);
-    if (p == nullptr || is_device_pointer(p)) return;
+    if (p == nullptr || (ranges ? ranges->is_device(p) : is_device_pointer(p))) return;
     const size_t es = dt_size(dt);
     s.host = const_cast<void*>(p);
     s.lo = lo * (int64_t)es;
@@ -167,11 +191,11 @@ static void stage_span(Staged& s, const void* p, Dt dt, int64_t lo, int64_t hi, 
 }
 
 void stage(Staged& s, const void* p, Dt dt, int64_t m, int64_t n, int64_t rs, int64_t cs,
-           bool is_output) {
+           bool is_output, DeviceRanges* ranges) {
     int64_t lo, hi;
     span_of(m, n, rs, cs, lo, hi);
     if (m <= 0 || n <= 0) { s.dev = const_cast<void*>(p); return; }
-    stage_span(s, p, dt, lo, hi, is_output);
+    stage_span(s, p, dt, lo, hi, is_output, ranges);
 }
 
 void stage_complex(Staged& s, const void* p, Dt dt, int64_t m, int64_t n, int64_t rs,

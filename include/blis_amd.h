@@ -60,10 +60,12 @@ int bli_amd_gemm_noalloc( num_t dt, trans_t transa, trans_t transb, dim_t m, dim
 
 /* ---- batched GEMM plans ------------------------------------------------
    A plan holds a batch (same arguments as bli_?gemm_batch, operands already
-   in device memory) as device descriptors and an LPT-ordered work list. Each
-   execute runs the whole batch as one persistent kernel; use_graph != 0
-   replays it as a HIP graph captured on first use. The plan keeps the
-   pointers it was built with. */
+   in device memory) as device descriptors and LPT-ordered work lists. Each
+   execute runs the batch as persistent kernels, one per tile variant (or,
+   when the autotuner finds it faster, as one dispatch whose workgroups take
+   tiles from per-XCD device counters); use_graph != 0 replays it as a HIP
+   graph captured on first use. The plan keeps the pointers it was built
+   with, and one plan's executions must not overlap. */
 void* bli_amd_dgemm_batch_plan_create( const trans_t* transa, const trans_t* transb,
                                        const dim_t* m, const dim_t* n, const dim_t* k,
                                        const double* alpha,
@@ -90,6 +92,11 @@ void  bli_amd_gemm_batch_plan_destroy( void* plan );
 int   bli_amd_gemm_batch_plan_autotune( void* plan, int reps );
 void  bli_amd_gemm_batch_plan_set_mode( void* plan, int mode );
 int   bli_amd_gemm_batch_plan_get_mode( void* plan );
+/* on != 0: one dispatch for all tile variants; 0 (the default): one
+   persistent kernel per variant (tile size x operand layouts), one after
+   another. The autotuner chooses between the two. */
+void  bli_amd_gemm_batch_plan_set_single_dispatch( void* plan, int on );
+int   bli_amd_gemm_batch_plan_get_single_dispatch( void* plan );
 
 /* ---- split-precision GEMM ----------------------------------------------
    Same arguments and semantics as bli_?gemm (C := beta*C + alpha*op(A)*op(B);
