@@ -508,10 +508,13 @@ def split_bench(bench, log, size=8192, sample_rows=8):
         flops = 2.0 * m * n * k
         tf_s = flops / (ms_split * 1e-3) / 1e12
         tf_n = flops / (ms_nat * 1e-3) / 1e12
-        prods = s * (s + 1) // 2
+        # s > 0: slices (s(s+1)/2 products); s < 0: -s moduli (one product each)
+        prods = s * (s + 1) // 2 if s > 0 else -s
         raw = prods * flops / (ms_split * 1e-3) / 1e12 if s else 0.0
         key = f"{ch}gemm_split_sigma{int(sigma)}_{m}"
-        out[key] = {"ms": round(ms_split, 4), "TFLOP/s": round(tf_s, 3), "slices": s,
+        out[key] = {"ms": round(ms_split, 4), "TFLOP/s": round(tf_s, 3),
+                    "scheme": "slices" if s > 0 else ("moduli" if s < 0 else "native"),
+                    "slices": s,
                     "products": prods, "int8_TOP/s": round(raw, 2),
                     "int8_frac": round(raw / 5000.0, 4),
                     "native_ms": round(ms_nat, 4), "native_TFLOP/s": round(tf_n, 3),

@@ -56,6 +56,7 @@ SOURCES = [
     "cblas.cpp",
     "cntx.cpp",
     "sup.hip",
+    "crt.hip",
 ]
 
 

@@ -66,7 +66,9 @@ BLIS_AMD_HIDDEN void        sync_if_needed();
 // copies of structured operands (symm/trmm) and trmm's copy of B, 22-24
 // complex GEMM (A in 1e format, B in 1r format, a column-major copy of C),
 // 25-27 mixed-datatype GEMM (A and B in the computation type, the product).
-constexpr int WS_SLOTS = 28;
+// 28-31 split precision's modular scheme (residues of A and of B, residue
+// products, constants).
+constexpr int WS_SLOTS = 32;
 BLIS_AMD_HIDDEN void*       workspace(size_t bytes, int slot);
 BLIS_AMD_HIDDEN bool        synchronous();   // calls wait for their results
 BLIS_AMD_HIDDEN bool        is_device_pointer(const void* p);
@@ -196,12 +198,22 @@ BLIS_AMD_HIDDEN int gemm_autotune(Dt dt, bool ta, bool tb, int64_t m, int64_t n,
 
 // ---- split precision (split.hip) ----
 constexpr int SPLIT_MAX_SLICES = 12;
+constexpr int CRT_MAX_MODULI = 32;
 struct SplitResult {
-    int    slices;      // INT8 slices per operand; 0 = the native path ran
+    int    slices;      // > 0: INT8 slices per operand (slice scheme); < 0: minus the
+                        // number of moduli (modular scheme, crt.hip); 0: native path
     double err_bound;   // a-priori Frobenius bound of the truncation error
 };
 BLIS_AMD_HIDDEN SplitResult gemm_split_run(const GemmDesc& d, int nslices, hipStream_t st);
 BLIS_AMD_HIDDEN int split_choose_slices(Dt dt, double gap_bits);
 BLIS_AMD_HIDDEN int split_max_slices(Dt dt);
+// Modular scheme (crt.hip): moduli and fractional bits for a gap and depth
+// (0 moduli: not applicable), the fractional bits n moduli support, and the
+// run itself on operand exponents already on the device (emax: m rows of
+// op(A), then n columns of op(B); h_emax the host copy).
+BLIS_AMD_HIDDEN int crt_choose_moduli(Dt dt, double gap_bits, int64_t k, int* sh_out);
+BLIS_AMD_HIDDEN int crt_shift_for(int n, int64_t k);
+BLIS_AMD_HIDDEN SplitResult crt_run(const GemmDesc& d, int* emax, const std::vector<int>& h_emax,
+                                    int n, int sh, hipStream_t st);
 
 } // namespace blis_amd

@@ -569,6 +569,10 @@ SplitResult gemm_split_run(const GemmDesc& d, int nslices, hipStream_t st) {
     }
     if (nslices > SPLIT_MAX_SLICES)
         abort_with(BLIS_INVALID_DIM_STRIDE_COMBINATION, "split gemm: more than 12 slices");
+    if (nslices < 0 && (nslices < -CRT_MAX_MODULI || nslices > -2))
+        abort_with(BLIS_INVALID_DIM_STRIDE_COMBINATION, "split gemm: 2 to 32 moduli");
+    if (nslices < 0 && d.k > ((int64_t)1 << 17))
+        abort_with(BLIS_INVALID_DIM_STRIDE_COMBINATION, "split gemm: modular scheme needs k <= 2^17");
 
     // 1. Exponent statistics of op(A)'s rows and op(B)'s columns.
     const int64_t rows = d.m + d.n;
@@ -589,14 +593,31 @@ SplitResult gemm_split_run(const GemmDesc& d, int nslices, hipStream_t st) {
     //    point take the native path (which propagates Inf/NaN as BLIS does).
     bool finite = true;
     for (int64_t i = 0; i < rows; ++i) finite &= (h_emax[i] != EXP_NONFINITE);
-    int S = nslices;
-    if (S <= 0) {
+    //    The automatic choice for FP64 takes the modular scheme when its
+    //    moduli are fewer than the slice scheme's products (crt.hip); a
+    //    negative count forces that scheme with -nslices moduli.
+    int S = nslices, crt_n = 0, crt_sh = 0;
+    if (nslices < 0) {
+        S = 0;
+        crt_n = -nslices;
+        crt_sh = crt_shift_for(crt_n, d.k);
+    } else if (S == 0) {
         const double gap = std::max(mean_gap(h_emax.data(), h_sum.data(), h_cnt.data(), d.m),
                                     mean_gap(h_emax.data() + d.m, h_sum.data() + d.m,
                                              h_cnt.data() + d.m, d.n));
         S = split_choose_slices(d.dt, gap);
         if (S > split_max_slices(d.dt)) S = 0;
+        if (d.dt == Dt::F64) {
+            int sh = 0;
+            const int n = crt_choose_moduli(d.dt, gap, d.k, &sh);
+            if (n > 0 && (S == 0 || n < S * (S + 1) / 2)) {
+                crt_n = n;
+                crt_sh = sh;
+                S = 0;
+            }
+        }
     }
+    if (finite && crt_n > 0) return crt_run(d, emax, h_emax, crt_n, crt_sh, st);
     if (!finite || S <= 0) {
         gemm_run(d, nullptr, st);
         return res;
@@ -716,6 +737,14 @@ int bli_amd_gemm_split_slices(int is_double, double gap_bits) {
 __attribute__((visibility("default")))
 int bli_amd_gemm_split_max_slices(int is_double) {
     return split_max_slices(is_double ? Dt::F64 : Dt::F32);
+}
+
+__attribute__((visibility("default")))
+int bli_amd_gemm_split_moduli(int is_double, double gap_bits, dim_t k, int* sh) {
+    int s = 0;
+    const int n = crt_choose_moduli(is_double ? Dt::F64 : Dt::F32, gap_bits, k, &s);
+    if (sh) *sh = s;
+    return n;
 }
 
 } // extern "C"

@@ -105,14 +105,22 @@ int   bli_amd_gemm_batch_plan_get_single_dispatch( void* plan );
    rows of op(A) and columns of op(B) are scaled by powers of two and split
    into 7-bit digits, slice products are accumulated exactly in int32 and
    summed in FP64).
-   nslices > 0 forces the slice count (at most 12). nslices <= 0 chooses it
-   from the operands' exponent range. If the operands hold Inf/NaN, or if the
-   chosen count is past the point where the emulation is slower than native
+   A second scheme (Ozaki scheme II) scales the rows and columns to integers
+   and forms their product exactly from its residues modulo up to 32
+   pairwise coprime moduli, one INT8 GEMM per modulus, rebuilt in FP64 by the
+   Chinese remainder theorem (k <= 2^17).
+   nslices > 0 forces the slice scheme with that many slices (at most 12);
+   nslices in [-32, -2] forces the modular scheme with -nslices moduli;
+   nslices == 0 chooses from the operands' exponent range: FP64 takes the
+   modular scheme when its moduli are fewer than the slice scheme's
+   products, FP32 the slice scheme. If the operands hold Inf/NaN, or if
+   the choice is past the point where the emulation is slower than native
    MFMA, the native bli_?gemm path runs instead.
-   Returns the slice count used (0 = native path). If err_bound is non-NULL,
-   it receives an a-priori bound on ||C_split - alpha*op(A)*op(B) - beta*C||_F
-   from the truncation (0 when the native path ran). The call waits for the
-   device to read back the operand statistics. */
+   Returns the slice count (> 0), minus the number of moduli (< 0), or 0
+   when the native path ran. If err_bound is non-NULL, it receives an
+   a-priori bound on ||C_split - alpha*op(A)*op(B) - beta*C||_F from the
+   truncation (0 when the native path ran). The call waits for the device
+   to read back the operand statistics. */
 int bli_amd_dgemm_split( trans_t transa, trans_t transb, dim_t m, dim_t n, dim_t k,
                          const double* alpha, const double* a, inc_t rs_a, inc_t cs_a,
                          const double* b, inc_t rs_b, inc_t cs_b, const double* beta,
@@ -127,6 +135,10 @@ int bli_amd_gemm_split_slices( int is_double, double gap_bits );
 /* Largest slice count the automatic policy uses before it falls back to the
    native path (the break-even point against native MFMA). */
 int bli_amd_gemm_split_max_slices( int is_double );
+/* Moduli the automatic policy uses for a mean exponent gap at depth k (0 when
+   the modular scheme does not apply), and the fractional bits kept per
+   operand entry (*sh, if non-NULL). */
+int bli_amd_gemm_split_moduli( int is_double, double gap_bits, dim_t k, int* sh );
 
 /* ---- configuration space, autotuner and tile selector -------------------
    A GEMM configuration is a tile of the gfx950 tile table combined with a

@@ -101,3 +101,141 @@ def gen_spread(rng: np.random.Generator, shape, sigma: float) -> np.ndarray:
         e = np.rint(rng.normal(0.0, sigma, shape)).astype(np.int64)
         return np.ldexp(u, e)
     return u
+
+
+# ---- the modular scheme (Ozaki scheme II) ---------------------------------------
+# Ozaki, Uchino, Imamura, "Ozaki Scheme II: A GEMM-oriented emulation of
+# floating-point matrix multiplication using an integer modular technique"
+# (2024). Same operations, in the same order, as the library's crt.hip:
+#
+#   a'_ip  = trunc(ldexp(a_ip, sh - e_i))          (integers, |a'| < 2^sh)
+#   r_l    = a' mod p_l as int8 (two's complement for 256, symmetric for odd p)
+#   c_l    = (R_A,l R_B,l) mod p_l                 (exact integer products)
+#   part_k = sum_l c_l w_lk                        (exact: 39-bit parts)
+#   q      = rint((part_0 + part_1) * inv_m)
+#   D_k    = part_k - q mpart_k                     (exact)
+#   X      = hi + (lo + tail): (hi, lo) = TwoSum(D_0, D_1), tail = sum of
+#            D_k for k = K-1 .. 2 in that order
+#   c_ij   = alpha * ldexp(X, e_i + f_j - 2 sh) (+ beta c_ij)
+
+MODULI = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191,
+          181, 179, 173, 167, 163, 157, 151, 149, 139, 137, 131, 127, 113, 109, 107, 103]
+PART_BITS = 39
+
+
+def crt_const(n: int) -> dict:
+    """Moduli, weights w_l = (M/p_l)((M/p_l)^-1 mod p_l) and M, each split into
+    39-bit parts (most significant first) as exact doubles, and 1/M."""
+    import math
+    ps = MODULI[:n]
+    M = math.prod(ps)
+    L = M.bit_length()
+    parts = -(-L // PART_BITS)
+    lo = [max(0, L - PART_BITS * (k + 1)) for k in range(parts)]
+    width = [L - PART_BITS * k - lo[k] for k in range(parts)]
+
+    def split(v: int):
+        return [math.ldexp(float((v >> lo[k]) & ((1 << width[k]) - 1)), lo[k])
+                for k in range(parts)]
+    w = []
+    for p in ps:
+        Ml = M // p
+        w.append(split(Ml * pow(Ml % p, -1, p)))
+    top_lo = max(0, L - 64)
+    inv_m = math.ldexp(1.0 / float(M >> top_lo), -top_lo)
+    return {"p": ps, "M": M, "w": w, "mpart": split(M), "inv_m": inv_m, "parts": parts}
+
+
+def crt_choose(is_double: bool, gap_bits: float, k: int):
+    """(moduli, sh): sh = ceil(significand bits + gap); the fewest moduli with
+    log2 M >= 2 sh + ceil(log2 k) + 2; (0, sh) if none suffice or k > 2^17."""
+    import math
+    sh = int(math.ceil((50.0 if is_double else 22.0) + gap_bits))
+    if k > (1 << 17):
+        return 0, sh
+    bits = 2.0 * sh + math.ceil(math.log2(max(k, 2))) + 2.0
+    acc = 0.0
+    for n, p in enumerate(MODULI, start=1):
+        acc += math.log2(p)
+        if n >= 2 and acc >= bits:
+            return n, sh
+    return 0, sh
+
+
+def crt_shift(n: int, k: int) -> int:
+    """Largest sh that n moduli support at depth k."""
+    import math
+    bits = sum(math.log2(p) for p in MODULI[:n]) - math.ceil(math.log2(max(k, 2))) - 2.0
+    return max(1, int(math.floor(bits / 2.0)))
+
+
+def crt_integers(x: np.ndarray, sh: int):
+    """trunc(ldexp(x_ip, sh - e_i)) per row (exact doubles) and the exponents."""
+    e = row_exponents(x)
+    return np.trunc(np.ldexp(x.astype(np.float64), (sh - e)[:, None])), e
+
+
+def crt_residues_of(ints: list, shape, p: int) -> np.ndarray:
+    """int8-range residues modulo p of Python integers (row-major, shape)."""
+    r = np.array([v % p for v in ints], dtype=np.int64).reshape(shape)
+    if p == 256:
+        return np.where(r >= 128, r - 256, r)
+    return np.where(r > (p - 1) // 2, r - p, r)
+
+
+def crt_residues(t: np.ndarray, p: int) -> np.ndarray:
+    """int8-range residues of the integer-valued doubles t modulo p."""
+    return crt_residues_of([int(v) for v in t.ravel()], t.shape, p)
+
+
+def crt_gemm(a_eff: np.ndarray, b_eff: np.ndarray, n: int, sh: int, alpha: float = 1.0,
+             beta: float = 0.0, c0: np.ndarray | None = None, exact_out: list | None = None):
+    """C = alpha op(A) op(B) (+ beta C0) by the modular scheme with n moduli
+    and sh fractional bits. exact_out, if given, receives the exact integer
+    products (Python ints) for checking the reconstruction."""
+    cc = crt_const(n)
+    ta, ea = crt_integers(a_eff, sh)
+    tb, eb = crt_integers(b_eff.T, sh)
+    m, k = ta.shape
+    nn = tb.shape[0]
+    ia = [int(v) for v in ta.ravel()]
+    ib = [int(v) for v in tb.ravel()]
+    prods = []
+    for p in cc["p"]:
+        ra, rb = crt_residues_of(ia, ta.shape, p), crt_residues_of(ib, tb.shape, p)
+        prods.append(np.mod(ra @ rb.T, p).astype(np.float64))
+    parts = [np.zeros((m, nn)) for _ in range(cc["parts"])]
+    for l, c in enumerate(prods):
+        for kk in range(cc["parts"]):
+            parts[kk] = parts[kk] + c * cc["w"][l][kk]        # exact
+    zero = np.zeros((m, nn))
+    q = np.rint((parts[0] + (parts[1] if cc["parts"] > 1 else zero)) * cc["inv_m"])
+    D = [parts[kk] - q * cc["mpart"][kk] for kk in range(cc["parts"])]   # each exact
+    d1 = D[1] if cc["parts"] > 1 else zero
+    hi = D[0] + d1                                             # TwoSum(D_0, D_1)
+    bb = hi - D[0]
+    lo = (D[0] - (hi - bb)) + (d1 - bb)
+    tail = zero
+    for kk in range(cc["parts"] - 1, 1, -1):
+        tail = tail + D[kk]
+    X = hi + (lo + tail)
+    if exact_out is not None:
+        ra_ = [ia[i * k:(i + 1) * k] for i in range(m)]
+        rb_ = [ib[j * k:(j + 1) * k] for j in range(nn)]
+        exact_out.append([[sum(x * y for x, y in zip(ra_[i], rb_[j])) for j in range(nn)]
+                          for i in range(m)])
+    val = alpha * np.ldexp(X, (ea[:, None] + eb[None, :] - 2 * sh))
+    if beta == 0.0 or c0 is None:
+        return val
+    return val + beta * c0
+
+
+def crt_bound(a_eff: np.ndarray, b_eff: np.ndarray, sh: int, alpha: float = 1.0) -> float:
+    """A-priori Frobenius bound |alpha| k 2^(1-sh) (1 + 2^-sh) ||2^e|| ||2^f||."""
+    ea = row_exponents(a_eff).astype(np.float64)
+    eb = row_exponents(b_eff.T).astype(np.float64)
+    nza = np.max(np.abs(a_eff), axis=1) > 0
+    nzb = np.max(np.abs(b_eff), axis=0) > 0
+    k = a_eff.shape[1]
+    return (abs(alpha) * k * 2.0 ** (1 - sh) * (1 + 2.0 ** -sh)
+            * np.sqrt(np.sum(np.exp2(2 * ea[nza]))) * np.sqrt(np.sum(np.exp2(2 * eb[nzb]))))

@@ -155,9 +155,16 @@ def test_policy_error_not_above_native(blis, oracle, sigma):
     a.to_device(); b.to_device(); c.to_device(); cn.to_device()
     gap = max(ozaki.mean_gap(eff(a, "n")), ozaki.mean_gap(eff(b, "n").T))
     s_pol = ozaki.choose_slices(True, gap)
+    # the automatic choice: the modular scheme when its moduli are fewer
+    # than the slice scheme's products (FP64), else the slice policy
+    n_crt, _ = ozaki.crt_choose(True, gap, k)
+    s_max = blis.bli_amd_gemm_split_max_slices(1)
+    want = s_pol if s_pol <= s_max else 0
+    if n_crt > 0 and (want == 0 or n_crt < want * (want + 1) // 2):
+        want = -n_crt
     auto, _ = split(blis, np.float64, "n", "n", m, n, k, 1.0, 0.0, a, b, c, 0)
-    assert auto == (s_pol if s_pol <= blis.bli_amd_gemm_split_max_slices(1) else 0)
-    # the policy's count, forced, must match native accuracy
+    assert auto == want
+    # the policy's slice count, forced, must match native accuracy
     got, bound = split(blis, np.float64, "n", "n", m, n, k, 1.0, 0.0, a, b, c, s_pol)
     assert got == s_pol
     native(blis, np.float64, "n", "n", m, n, k, 1.0, 0.0, a, b, cn)
@@ -171,9 +178,11 @@ def test_policy_error_not_above_native(blis, oracle, sigma):
 
 
 def test_wide_spread_falls_back_to_native(blis):
+    # sigma = 40: entries spread over about 120 bits below their row maxima,
+    # past both 8 slices and 32 moduli
     m, n, k = 100, 70, 512
-    say("split fallback sigma=12")
-    a, b, c = operands(np.float64, "n", "n", m, n, k, 5, "col", sigma=12.0)
+    say("split fallback sigma=40")
+    a, b, c = operands(np.float64, "n", "n", m, n, k, 5, "col", sigma=40.0)
     cn = Mat(m, n, seed=3)
     a.to_device(); b.to_device(); c.to_device(); cn.to_device()
     got, bound = split(blis, np.float64, "n", "n", m, n, k, 1.0, 0.0, a, b, c, 0)
@@ -234,3 +243,52 @@ def test_host_pointers_are_staged(blis):
                                    C.byref(eb))
     assert got == 8
     np.testing.assert_array_equal(c.view(), ozaki.split_gemm(eff(a, "n"), eff(b, "n"), 8))
+
+
+# ---- the modular scheme (Ozaki scheme II, nslices < 0) ---------------------------
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("ta,tb", [("n", "n"), ("t", "n"), ("n", "t")])
+@pytest.mark.parametrize("layout", ["col", "row", "gen"])
+def test_modular_bit_identical_to_model(blis, dtype, ta, tb, layout):
+    """Forced moduli: the result equals the CPU model (oracle/ozaki.py
+    crt_gemm) bit for bit, with beta = 0 and beta != 0, across two output
+    tiles of 256 in each direction and a partial k chunk."""
+    m, n, k = 300, 270, 130
+    nmod = 16 if np.dtype(dtype) == np.float64 else 9
+    for beta in (0.0, 0.5):
+        say(f"split modular {np.dtype(dtype).name} {ta}{tb} {layout} beta={beta} {m}x{n}x{k} "
+            f"moduli={nmod}")
+        a, b, c = operands(dtype, ta, tb, m, n, k, 31, layout)
+        c0 = c.view().astype(np.float64).copy()
+        a.to_device(); b.to_device(); c.to_device()
+        got, bound = split(blis, dtype, ta, tb, m, n, k, 0.75, beta, a, b, c, -nmod)
+        assert got == -nmod and bound > 0
+        sh = ozaki.crt_shift(nmod, k)
+        ref = ozaki.crt_gemm(eff(a, ta), eff(b, tb), nmod, sh, alpha=0.75, beta=beta, c0=c0)
+        np.testing.assert_array_equal(c.device_view(), ref.astype(dtype))
+
+
+@pytest.mark.parametrize("sigma", [0.0, 4.0, 8.0])
+def test_modular_policy_error_not_above_native(blis, oracle, sigma):
+    """The automatic FP64 choice on u * 2^round(N(0, sigma)) data takes the
+    modular scheme with the policy's moduli; its error against the
+    double-double oracle is no larger than native bli_dgemm's and within the
+    reported bound plus FP64 rounding."""
+    m, n, k = 128, 96, 4096
+    say(f"split modular policy sigma={sigma} {m}x{n}x{k}")
+    a, b, c = operands(np.float64, "n", "n", m, n, k, 78, "col", sigma=sigma)
+    cn = Mat(m, n, seed=2)
+    a.to_device(); b.to_device(); c.to_device(); cn.to_device()
+    gap = max(ozaki.mean_gap(eff(a, "n")), ozaki.mean_gap(eff(b, "n").T))
+    n_crt, sh = ozaki.crt_choose(True, gap, k)
+    got, bound = split(blis, np.float64, "n", "n", m, n, k, 1.0, 0.0, a, b, c, 0)
+    assert n_crt > 0 and got == -n_crt
+    native(blis, np.float64, "n", "n", m, n, k, 1.0, 0.0, a, b, cn)
+    ae, be = eff(a, "n"), eff(b, "n")
+    ref = dd_ref(oracle, ae, be)
+    e_split = np.linalg.norm(c.device_view() - ref)
+    e_nat = np.linalg.norm(cn.device_view() - ref)
+    say(f"  moduli={n_crt} sh={sh} split_err={e_split:.3e} native_err={e_nat:.3e} bound={bound:.3e}")
+    assert e_split <= e_nat
+    assert e_split <= bound + 8 * 2.0 ** -53 * np.linalg.norm(np.abs(ae) @ np.abs(be))

@@ -131,3 +131,106 @@ def test_library_policy_agrees_with_model():
         for gap in (0.0, 1.44, 3.0, 7.9, 16.5, 31.4, 60.0):
             assert L.bli_amd_gemm_split_slices(is_double, C.c_double(gap)) == \
                 ozaki.choose_slices(bool(is_double), gap)
+
+
+# ---- the modular scheme (Ozaki scheme II) ----------------------------------------
+
+def test_crt_constants_are_crt_weights():
+    """w_l = 1 mod p_l and 0 mod every other modulus, w_l < M, the 39-bit
+    parts of w_l and of M add back to them exactly, and the moduli are
+    pairwise coprime."""
+    import math
+    for n in (2, 9, 16, 24, 32):
+        cc = ozaki.crt_const(n)
+        ps, M = cc["p"], cc["M"]
+        for i in range(n):
+            for j in range(i + 1, n):
+                assert math.gcd(ps[i], ps[j]) == 1
+        for l, parts in enumerate(cc["w"]):
+            w = sum(int(v) for v in parts)
+            assert 0 <= w < M
+            for j, p in enumerate(ps):
+                assert w % p == (1 if j == l else 0)
+            assert all(float(v).is_integer() and v < 2.0 ** (M.bit_length() + 1) for v in parts)
+        assert sum(int(v) for v in cc["mpart"]) == M
+
+
+@pytest.mark.parametrize("sigma", [0.0, 4.0, 8.0])
+def test_crt_reconstructs_the_exact_integer_product(sigma):
+    """The FP64 reconstruction gives the exact integer product of the
+    operands' integer images (Python integers) to within one rounding."""
+    rng = np.random.default_rng(int(sigma) + 40)
+    a = ozaki.gen_spread(rng, (7, 90), sigma)
+    b = ozaki.gen_spread(rng, (90, 6), sigma)
+    a[2, :] = 0.0
+    gap = max(ozaki.mean_gap(a), ozaki.mean_gap(b.T))
+    n, sh = ozaki.crt_choose(True, gap, 90)
+    assert n > 0
+    exact = []
+    c = ozaki.crt_gemm(a, b, n, sh, exact_out=exact)
+    _, ea = ozaki.crt_integers(a, sh)
+    _, eb = ozaki.crt_integers(b.T, sh)
+    X = np.ldexp(c, -(ea[:, None] + eb[None, :] - 2 * sh))
+    for i in range(7):
+        for j in range(6):
+            e = exact[0][i][j]
+            assert abs(int(X[i, j]) - e) <= max(1, abs(e)) * 2.0 ** -52, (i, j)
+    assert np.all(c[2] == 0.0)
+
+
+@pytest.mark.parametrize("sigma", [0.0, 4.0, 8.0])
+def test_crt_within_bound_and_native_error(oracle, sigma):
+    """The policy's moduli and shift: error vs the double-double oracle within
+    the a-priori bound (plus FP64 rounding) and no larger than native GEMM's."""
+    rng = np.random.default_rng(int(sigma) + 200)
+    m = n = 16
+    k = 2048
+    a = ozaki.gen_spread(rng, (m, k), sigma)
+    b = ozaki.gen_spread(rng, (k, n), sigma)
+    ref = _dd(oracle, a, b)
+    nat = np.zeros((m, n), order="F")
+    oracle.gemm("n", "n", 1.0, np.asfortranarray(a), np.asfortranarray(b), 0.0, nat)
+    gap = max(ozaki.mean_gap(a), ozaki.mean_gap(b.T))
+    nm, sh = ozaki.crt_choose(True, gap, k)
+    got = ozaki.crt_gemm(a, b, nm, sh)
+    rnd = 4 * 2.0 ** -53 * np.linalg.norm(np.abs(a) @ np.abs(b))
+    assert np.linalg.norm(got - ref) <= ozaki.crt_bound(a, b, sh) + rnd
+    den = np.linalg.norm(a) * np.linalg.norm(b)
+    assert np.linalg.norm(got - ref) / den <= np.linalg.norm(nat - ref) / den
+
+
+def test_crt_planted_mistake_too_few_moduli(oracle):
+    """Two moduli fewer than the shift needs (M below 4 k 2^(2 sh)) wraps the
+    product around M: the result leaves the bound by orders of magnitude."""
+    rng = np.random.default_rng(9)
+    a = rng.uniform(-1, 1, (10, 300))
+    b = rng.uniform(-1, 1, (300, 8))
+    ref = _dd(oracle, a, b)
+    n, sh = ozaki.crt_choose(True, 1.44, 300)
+    good = ozaki.crt_gemm(a, b, n, sh)
+    bad = ozaki.crt_gemm(a, b, n - 2, sh)
+    bound = ozaki.crt_bound(a, b, sh)
+    assert np.linalg.norm(good - ref) <= bound + 4 * 2.0 ** -53 * np.linalg.norm(np.abs(a) @ np.abs(b))
+    assert np.linalg.norm(bad - ref) > 1e6 * bound
+
+
+def test_crt_policy_counts():
+    # uniform [-1,1] data at k = 8192: sh = ceil(50 + 1.44) = 52 needs
+    # 2*52 + 13 + 2 = 119 bits of M; 15 moduli give 117.6 bits, 16 give 125.2
+    assert ozaki.crt_choose(True, 1.0 / np.log(2), 8192) == (16, 52)
+    assert ozaki.crt_choose(True, 0.0, 1 << 18) == (0, 50)
+    # and the shift supported by n moduli inverts that rule
+    assert ozaki.crt_shift(16, 8192) >= 52 and ozaki.crt_shift(15, 8192) < 52
+
+
+def test_library_crt_policy_agrees_with_model():
+    import blis_amd
+    L = blis_amd.lib()
+    L.bli_amd_gemm_split_moduli.argtypes = [C.c_int, C.c_double, C.c_int64, C.POINTER(C.c_int)]
+    L.bli_amd_gemm_split_moduli.restype = C.c_int
+    for is_double in (0, 1):
+        for gap in (0.0, 1.44, 3.0, 7.9, 16.5, 31.4, 60.0, 120.0):
+            for k in (64, 4096, 8192, 100000, 1 << 18):
+                sh = C.c_int(-1)
+                n = L.bli_amd_gemm_split_moduli(is_double, gap, k, C.byref(sh))
+                assert (n, sh.value) == ozaki.crt_choose(bool(is_double), gap, k), (gap, k)
