@@ -49,7 +49,7 @@ constexpr int TILE = 256;       // rows of X and of Y per workgroup
 constexpr int NT = 512;         // eight waves
 constexpr int ROWS_ALIGN = 256; // residue planes are padded to whole tiles
 constexpr int PARTS_MAX = 7;    // 39-bit parts of a weight (up to 273 bits)
-constexpr int POW2_MAX = 64;    // 2^e mod p for e < 64
+constexpr int POW2_MAX = 128;   // 2^e mod p for e < 128 (sh < 181)
 constexpr int PART_BITS = 39;
 
 // Pairwise coprime moduli, largest first: 2^8, and odd numbers whose prime
@@ -64,7 +64,7 @@ struct CrtConst {
     int n;                              // moduli
     int parts;                          // parts of each weight
     int p[CRT_MAX_MODULI];
-    double inv_p[CRT_MAX_MODULI];       // 1/p rounded
+    double inv_p[CRT_MAX_MODULI];       // 1/p in FP64
     double w[CRT_MAX_MODULI][PARTS_MAX];// w_l in parts, most significant first
     double mpart[PARTS_MAX];            // M in the same parts
     double inv_m;                       // 1/M rounded
@@ -83,33 +83,23 @@ __device__ inline double fmod_p(double v, double p, double ip) {
     return r;
 }
 
-// An integer-valued double t = sign * mant * 2^ex with mant < 2^53 an
-// integer (ex = 0 when |t| < 2^53).
-struct IntRep {
-    double mant;
-    int ex;
-    bool neg;
-};
-__device__ inline IntRep int_rep(double t) {
-    IntRep v{fabs(t), 0, t < 0.0};
-    int E;
-    (void)frexp(v.mant, &E);            // |t| = f 2^E, 0.5 <= f < 1
-    if (E > 53) {
-        v.ex = E - 53;
-        v.mant = ldexp(v.mant, -v.ex);  // exact: t has at most 53 significant bits
-    }
-    return v;
+// The symmetric residue of an integer-valued double |v| < 2^53 modulo p
+// (in [-(p-1)/2, (p-1)/2], or [-128, 128] for 256, whose byte is the same
+// for +-128): v - rint(v/p) p is exact by fma and lies within (0.52 p) of 0
+// because v/p is rounded by at most 2^-52 relative; a second step with
+// |r/p| < 0.52 is exact. FP64 FMA runs at the int32 rate on this GPU.
+__device__ inline double sym_mod(double v, double p, double ip) {
+    double r = fma(-rint(v * ip), p, v);
+    return fma(-rint(r * ip), p, r);
 }
 
-// The int8 residue of t modulo p_l.
-__device__ inline signed char residue(const IntRep& v, const CrtConst& c, int l) {
-    const double p = (double)c.p[l], ip = c.inv_p[l];
-    double r = fmod_p(v.mant, p, ip);
-    r = fmod_p(r * (double)c.pow2[l][v.ex], p, ip);   // r * 2^ex mod p (product < 2^16)
-    int ri = (int)r;
-    if (v.neg && ri != 0) ri = c.p[l] - ri;
-    if (c.p[l] == 256) return (signed char)(ri >= 128 ? ri - 256 : ri);
-    return (signed char)(ri > (c.p[l] - 1) / 2 ? ri - c.p[l] : ri);
+// An integer-valued double t as t = vm 2^ex with |vm| < 2^53 (ex = 0 when
+// |t| < 2^53, ex < 128 since |t| < 2^sh, sh < 181).
+__device__ inline double int_split(double t, int& ex) {
+    int E;
+    (void)frexp(t, &E);   // |t| = f 2^E, 0.5 <= f < 1
+    ex = (E > 53) ? E - 53 : 0;
+    return ex ? ldexp(t, -ex) : t;   // exact: t has at most 53 significant bits
 }
 
 // Residue planes of X_eff (rows x K), X_eff(i, p) = x[i*s_row + p*s_k], in
@@ -117,7 +107,10 @@ __device__ inline signed char residue(const IntRep& v, const CrtConst& c, int l)
 //   out[l*plane + (kc*rows_pad + i)*64 + (p % 64)],  kc = p / 64,
 // so the k chunk of a 256-row tile is one contiguous 16 KiB block. Rows >=
 // rows and p >= K are zero. A workgroup handles 64 rows x 64 k at a time:
-// lanes along rows when rows are adjacent in memory, along k otherwise.
+// the integers trunc(x 2^(sh - e_i)) are staged in LDS (read with lanes
+// along whichever index is adjacent in memory), then thread (r, sub) takes
+// k positions sub*16 .. sub*16 + 15 of row r and writes them as one 16-byte
+// chunk per plane.
 template <typename T>
 __global__ __launch_bounds__(256) void crt_residue_kernel(const T* __restrict__ x, int64_t rows,
                                                           int64_t rows_pad, int64_t K,
@@ -127,44 +120,28 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(const T* __restrict__ 
                                                           const CrtConst* __restrict__ cc,
                                                           signed char* __restrict__ out,
                                                           int64_t plane) {
-    __shared__ CrtConst c;
-    for (int i = threadIdx.x; i < (int)(sizeof(CrtConst) / 4); i += blockDim.x)
-        reinterpret_cast<int*>(&c)[i] = reinterpret_cast<const int*>(cc)[i];
-    __syncthreads();
-    const int tid = threadIdx.x;
+    __shared__ double tt[64][65];
+    __shared__ unsigned char pw[CRT_MAX_MODULI][POW2_MAX];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int n = cc->n;
+    for (int i = tid; i < CRT_MAX_MODULI * POW2_MAX / 4; i += blockDim.x)
+        reinterpret_cast<uint32_t*>(&pw[0][0])[i] = reinterpret_cast<const uint32_t*>(&cc->pow2[0][0])[i];
     const int64_t rblocks = rows_pad / 64, tiles = rblocks * nkc;
+    // Output mapping: a wave writes 16 whole rows (1 KiB) per plane.
+    const int r = w * 16 + (lane >> 2), sub = lane & 3;
     for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const int64_t rb = tile % rblocks, kc = tile / rblocks;
         if (row_contig) {
-            // Thread (r, sub): k positions sub*16 .. sub*16 + 15 of row r,
-            // written as one 16-byte chunk per plane.
-            const int r = tid & 63, sub = tid >> 6;
-            const int64_t i = rb * 64 + r;
-            IntRep v[16];
-            if (i < rows) {
-                const int em = emax[i];
-                const int e = (em == INT_MIN) ? 0 : em;
-                const int64_t pb = kc * KC + sub * 16;
-#pragma unroll
-                for (int q = 0; q < 16; ++q) {
-                    const int64_t pk = pb + q;
-                    v[q] = int_rep((pk < K) ? trunc(ldexp((double)x[i * s_row + pk * s_k], sh - e))
-                                            : 0.0);
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < 16; ++q) v[q] = int_rep(0.0);
-            }
-            signed char* dst = out + (kc * rows_pad + i) * KC + sub * 16;
-            for (int l = 0; l < c.n; ++l) {
-                union { signed char b[16]; i4_t w; } dg;
-#pragma unroll
-                for (int q = 0; q < 16; ++q) dg.b[q] = residue(v[q], c, l);
-                *reinterpret_cast<i4_t*>(dst + l * plane) = dg.w;
+            const int64_t i = rb * 64 + lane;
+            const int em = (i < rows) ? emax[i] : INT_MIN;
+            const int e = (em == INT_MIN) ? 0 : em;
+#pragma unroll 4
+            for (int q = 0; q < 16; ++q) {
+                const int64_t pk = kc * KC + w * 16 + q;
+                tt[lane][w * 16 + q] =
+                    (i < rows && pk < K) ? trunc(ldexp((double)x[i * s_row + pk * s_k], sh - e)) : 0.0;
             }
         } else {
-            // Lane l: k position l of rows w*16 .. w*16 + 15 (w = wave).
-            const int lane = tid & 63, w = tid >> 6;
             const int64_t pk = kc * KC + lane;
 #pragma unroll 4
             for (int q = 0; q < 16; ++q) {
@@ -175,10 +152,43 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(const T* __restrict__ 
                     const int e = (em == INT_MIN) ? 0 : em;
                     t = trunc(ldexp((double)x[i * s_row + pk * s_k], sh - e));
                 }
-                const IntRep v = int_rep(t);
-                signed char* dst = out + (kc * rows_pad + i) * KC + lane;
-                for (int l = 0; l < c.n; ++l) dst[l * plane] = residue(v, c, l);
+                tt[w * 16 + q][lane] = t;
             }
+        }
+        __syncthreads();
+        double vm[16];
+        int ex[16];
+        int any = 0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            vm[q] = int_split(tt[r][sub * 16 + q], ex[q]);
+            any |= ex[q];
+        }
+        __syncthreads();   // tt is free for the next tile
+        const bool big = __any(any != 0);   // some integer of the wave is >= 2^53
+        signed char* dst = out + (kc * rows_pad + rb * 64 + r) * KC + sub * 16;
+        for (int l = 0; l < n; ++l) {
+            const double p = (double)cc->p[l], ip = cc->inv_p[l];
+            int res[16];
+            if (!big) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) res[q] = (int)sym_mod(vm[q], p, ip);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const double r0 = sym_mod(vm[q], p, ip);
+                    // times 2^ex mod p: |r0 pw| < 2^16, one exact step
+                    res[q] = (int)sym_mod(r0 * (double)pw[l][ex[q]], p, ip);
+                }
+            }
+            uint32_t wd[4];
+#pragma unroll
+            for (int b4 = 0; b4 < 4; ++b4)
+                wd[b4] = (uint32_t)(res[4 * b4] & 0xff) | ((uint32_t)(res[4 * b4 + 1] & 0xff) << 8) |
+                         ((uint32_t)(res[4 * b4 + 2] & 0xff) << 16) |
+                         ((uint32_t)(res[4 * b4 + 3] & 0xff) << 24);
+            *reinterpret_cast<i4_t*>(dst + l * plane) =
+                i4_t{(int)wd[0], (int)wd[1], (int)wd[2], (int)wd[3]};
         }
     }
 }
@@ -314,54 +324,104 @@ __global__ __launch_bounds__(NT) void crt_gemm_kernel(CrtGemmArgs g) {
 // ---- reconstruction ----------------------------------------------------------
 // C(x, y) = alpha 2^(ex_x + ey_y - 2 sh) X + beta C(x, y), X rebuilt from the
 // residue products (step 3 of the file comment). beta == 0 does not read C.
-template <typename T>
+// A thread rebuilds four adjacent y of one x (one 4-byte load per plane when
+// vec4 is set: ld and plane multiples of 4); blockIdx.y walks x. The moduli's
+// constants are wave-uniform and come through scalar loads. PARTS >= 2: the
+// weights of a one-part modulus set carry a zero second part.
+template <typename T, int PARTS>
 __global__ __launch_bounds__(256) void crt_combine_kernel(const unsigned char* __restrict__ prod,
                                                           int64_t plane, int64_t ld,
-                                                          int64_t nx, int64_t ny,
+                                                          int64_t nx, int64_t ny, int vec4,
                                                           const CrtConst* __restrict__ cc,
                                                           const int* __restrict__ ex,
                                                           const int* __restrict__ ey, int sh,
                                                           T* C, int64_t c_sx, int64_t c_sy,
                                                           double alpha, double beta) {
-    __shared__ CrtConst c;
-    for (int i = threadIdx.x; i < (int)(sizeof(CrtConst) / 4); i += blockDim.x)
-        reinterpret_cast<int*>(&c)[i] = reinterpret_cast<const int*>(cc)[i];
-    __syncthreads();
-    const int64_t total = nx * ny;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t y = e % ny, x = e / ny;
-        double part[PARTS_MAX];
+    const int n = cc->n;
+    const int64_t y0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (y0 >= ny) return;
+    const int ny_here = (int)((ny - y0 < 4) ? ny - y0 : 4);
+    for (int64_t x = blockIdx.y; x < nx; x += gridDim.y) {
+        double part[4][PARTS];
 #pragma unroll
-        for (int k = 0; k < PARTS_MAX; ++k) part[k] = 0.0;
-        for (int l = 0; l < c.n; ++l) {
-            const double cl = (double)prod[l * plane + x * ld + y];
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
-            for (int k = 0; k < PARTS_MAX; ++k)
-                if (k < c.parts) part[k] = fma(cl, c.w[l][k], part[k]);   // exact
+            for (int k = 0; k < PARTS; ++k) part[j][k] = 0.0;
+        const unsigned char* src = prod + x * ld + y0;
+        for (int l = 0; l < n; ++l) {
+            uint32_t v;
+            if (vec4 && ny_here == 4) {
+                v = *reinterpret_cast<const uint32_t*>(src + l * plane);
+            } else {
+                v = 0;
+                for (int j = 0; j < ny_here; ++j) v |= (uint32_t)src[l * plane + j] << (8 * j);
+            }
+            double wl[PARTS];
+#pragma unroll
+            for (int k = 0; k < PARTS; ++k) wl[k] = cc->w[l][k];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const double cl = (double)((v >> (8 * j)) & 0xffu);
+#pragma unroll
+                for (int k = 0; k < PARTS; ++k) part[j][k] = fma(cl, wl[k], part[j][k]);   // exact
+            }
         }
-        const double q = rint((part[0] + part[1]) * c.inv_m);
-        // D_k = part_k - q mpart_k, each exact. D_0 and D_1 overlap and may
-        // cancel, so they are added error-free (TwoSum); the smaller parts
-        // are added to the error term, smallest first.
-        double D[PARTS_MAX];
-#pragma unroll
-        for (int k = 0; k < PARTS_MAX; ++k) D[k] = fma(-q, c.mpart[k], part[k]);
-        const double hi = D[0] + D[1];
-        const double bb = hi - D[0];
-        double lo = (D[0] - (hi - bb)) + (D[1] - bb);
-        double tail = 0.0;
-#pragma unroll
-        for (int k = PARTS_MAX - 1; k >= 2; --k)
-            if (k < c.parts) tail += D[k];
-        lo += tail;
-        const double X = hi + lo;
         const int fx = ex[x] == INT_MIN ? 0 : ex[x];
-        const int fy = ey[y] == INT_MIN ? 0 : ey[y];
-        const double val = alpha * ldexp(X, fx + fy - 2 * sh);
-        T* dst = C + x * c_sx + y * c_sy;
-        *dst = (T)(beta == 0.0 ? val : val + beta * (double)*dst);
+        const double inv_m = cc->inv_m;
+        double mp[PARTS];
+#pragma unroll
+        for (int k = 0; k < PARTS; ++k) mp[k] = cc->mpart[k];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= ny_here) break;
+            const double q = rint((part[j][0] + part[j][1]) * inv_m);
+            // D_k = part_k - q mpart_k, each exact. D_0 and D_1 overlap and
+            // may cancel, so they are added error-free (TwoSum); the smaller
+            // parts are added to the error term, smallest first.
+            double D[PARTS];
+#pragma unroll
+            for (int k = 0; k < PARTS; ++k) D[k] = fma(-q, mp[k], part[j][k]);
+            const double hi = D[0] + D[1];
+            const double bb = hi - D[0];
+            double lo = (D[0] - (hi - bb)) + (D[1] - bb);
+            double tail = 0.0;
+#pragma unroll
+            for (int k = PARTS - 1; k >= 2; --k) tail += D[k];
+            lo += tail;
+            const double X = hi + lo;
+            const int64_t y = y0 + j;
+            const int fy = ey[y] == INT_MIN ? 0 : ey[y];
+            const double val = alpha * ldexp(X, fx + fy - 2 * sh);
+            T* dst = C + x * c_sx + y * c_sy;
+            *dst = (T)(beta == 0.0 ? val : val + beta * (double)*dst);
+        }
     }
+}
+
+template <typename T>
+void launch_combine(const unsigned char* prod, int64_t plane, int64_t ld, int64_t nx,
+                    int64_t ny, int parts, const CrtConst* cc, const int* ex, const int* ey,
+                    int sh, T* C, int64_t c_sx, int64_t c_sy, double alpha, double beta,
+                    hipStream_t st) {
+    const int vec4 = (ld % 4 == 0 && plane % 4 == 0) ? 1 : 0;
+    const dim3 grid((unsigned)((ny + 1023) / 1024), (unsigned)std::min<int64_t>(nx, 65535));
+    switch (std::max(parts, 2)) {
+#define BLIS_AMD_CRT_COMBINE(P)                                                                 \
+    case P:                                                                                     \
+        hipLaunchKernelGGL((crt_combine_kernel<T, P>), grid, dim3(256), 0, st, prod, plane, ld, \
+                           nx, ny, vec4, cc, ex, ey, sh, C, c_sx, c_sy, alpha, beta);           \
+        break;
+        BLIS_AMD_CRT_COMBINE(2)
+        BLIS_AMD_CRT_COMBINE(3)
+        BLIS_AMD_CRT_COMBINE(4)
+        BLIS_AMD_CRT_COMBINE(5)
+        BLIS_AMD_CRT_COMBINE(6)
+        BLIS_AMD_CRT_COMBINE(7)
+#undef BLIS_AMD_CRT_COMBINE
+    default:
+        abort_with(BLIS_NOT_YET_IMPLEMENTED, "crt: weight parts out of range");
+    }
+    HIP_CHECK(hipGetLastError());
 }
 
 // ---- host: moduli, weights and their parts --------------------------------------
@@ -461,8 +521,9 @@ void build_const(int n, CrtConst& c) {
     }
 }
 
-// The device copy of the constants for n moduli (rebuilt when n changes).
-const CrtConst* device_const(int n, hipStream_t st) {
+// The device copy of the constants for n moduli (rebuilt when n changes);
+// *parts receives the number of weight parts.
+const CrtConst* device_const(int n, hipStream_t st, int* parts) {
     static int have = -1;
     static CrtConst host;
     void* dev = workspace(sizeof(CrtConst), 31);
@@ -472,12 +533,8 @@ const CrtConst* device_const(int n, hipStream_t st) {
         HIP_CHECK(hipStreamSynchronize(st));
         have = n;
     }
+    *parts = host.parts;
     return (const CrtConst*)dev;
-}
-
-inline unsigned grid1(int64_t threads) {
-    const int64_t g = (threads + 255) / 256;
-    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 16384));
 }
 
 template <typename T>
@@ -521,7 +578,8 @@ int crt_shift_for(int n, int64_t k) {
 SplitResult crt_run(const GemmDesc& d, int* emax, const std::vector<int>& h_emax, int n, int sh,
                     hipStream_t st) {
     SplitResult res{-n, 0.0};
-    const CrtConst* cc = device_const(n, st);
+    int parts = 0;
+    const CrtConst* cc = device_const(n, st, &parts);
     // A-priori bound ||C - C~||_F <= |alpha| k 2^(1 - sh) (1 + 2^-sh)
     //   * sqrt(sum_i 4^e_i) * sqrt(sum_j 4^f_j).
     auto sum4 = [&](int64_t off, int64_t cnt) {
@@ -582,14 +640,11 @@ SplitResult crt_run(const GemmDesc& d, int* emax, const std::vector<int>& h_emax
 
     // Reconstruction into C.
     if (d.dt == Dt::F64)
-        hipLaunchKernelGGL(crt_combine_kernel<double>, dim3(grid1(g.nx * g.ny)), dim3(256), 0,
-                           st, g.out, g.out_plane, g.ld_out, g.nx, g.ny, cc, ex, ey, sh,
-                           (double*)d.c.ptr, c_sx, c_sy, d.alpha, d.beta);
+        launch_combine<double>(g.out, g.out_plane, g.ld_out, g.nx, g.ny, parts, cc, ex, ey, sh,
+                               (double*)d.c.ptr, c_sx, c_sy, d.alpha, d.beta, st);
     else
-        hipLaunchKernelGGL(crt_combine_kernel<float>, dim3(grid1(g.nx * g.ny)), dim3(256), 0,
-                           st, g.out, g.out_plane, g.ld_out, g.nx, g.ny, cc, ex, ey, sh,
-                           (float*)d.c.ptr, c_sx, c_sy, d.alpha, d.beta);
-    HIP_CHECK(hipGetLastError());
+        launch_combine<float>(g.out, g.out_plane, g.ld_out, g.nx, g.ny, parts, cc, ex, ey, sh,
+                              (float*)d.c.ptr, c_sx, c_sy, d.alpha, d.beta, st);
     return res;
 }
 

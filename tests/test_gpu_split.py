@@ -269,6 +269,26 @@ def test_modular_bit_identical_to_model(blis, dtype, ta, tb, layout):
         np.testing.assert_array_equal(c.device_view(), ref.astype(dtype))
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("layout", ["col", "row"])
+def test_modular_all_moduli_bit_identical(blis, dtype, layout):
+    """All 32 moduli at depth 8: the shift is 117 bits, so the largest
+    scaled integers have 117 bits and their residues need 2^e mod p for
+    e = 64; the result still equals the CPU model bit for bit."""
+    m, n, k = 70, 66, 8
+    nmod = 32
+    sh = ozaki.crt_shift(nmod, k)
+    assert sh > 116
+    say(f"split modular all moduli {np.dtype(dtype).name} {layout} {m}x{n}x{k} sh={sh}")
+    a, b, c = operands(dtype, "n", "n", m, n, k, 32, layout)
+    c0 = c.view().astype(np.float64).copy()
+    a.to_device(); b.to_device(); c.to_device()
+    got, bound = split(blis, dtype, "n", "n", m, n, k, 1.25, 0.5, a, b, c, -nmod)
+    assert got == -nmod and bound > 0
+    ref = ozaki.crt_gemm(eff(a, "n"), eff(b, "n"), nmod, sh, alpha=1.25, beta=0.5, c0=c0)
+    np.testing.assert_array_equal(c.device_view(), ref.astype(dtype))
+
+
 @pytest.mark.parametrize("sigma", [0.0, 4.0, 8.0])
 def test_modular_policy_error_not_above_native(blis, oracle, sigma):
     """The automatic FP64 choice on u * 2^round(N(0, sigma)) data takes the
