@@ -444,10 +444,10 @@ def struc_bench(bench, rec, n=8192):
 def split_bench(bench, log, size=8192, sample_rows=8):
     """Split-precision GEMM on u * 2^round(N(0, sigma)) operands.
 
-    For each sigma, the slice count is chosen by the library's policy
-    (nslices = 0). Reported per case:
-      * end-to-end ms (statistics, read-back, slicing, slice GEMM) and
-        effective 2mnk TFLOP/s;
+    For each sigma, the scheme and its slice or moduli count are chosen by
+    the library's policy (nslices = 0). Reported per case:
+      * end-to-end ms (statistics, read-back, slices or residues, the INT8
+        products, reconstruction) and effective 2mnk TFLOP/s;
       * the raw INT8 rate, products * 2mnk, against the 5.0 POP/s INT8 peak;
       * the normwise error of split and of native GEMM, both against the
         double-double oracle on `sample_rows` rows of C, measured relative
@@ -526,6 +526,19 @@ def split_bench(bench, log, size=8192, sample_rows=8):
             f"native {err_n:.2e}")
         del a, b, cs, cn
         torch.cuda.empty_cache()
+    # Reference point for the INT8 rate: the vendor INT8 GEMM (torch._int_mm,
+    # hipBLASLt) on random int8 operands of the same size, B transposed and not.
+    ia = torch.randint(-100, 100, (m, k), dtype=torch.int8, device="cuda")
+    ib = torch.randint(-100, 100, (n, k), dtype=torch.int8, device="cuda")
+    ibn = ib.t().contiguous()
+    vendor = {}
+    for lay, bb in (("nt", ib.t()), ("nn", ibn)):
+        ms = bench.kernel_ms(lambda: torch._int_mm(ia, bb), 3)
+        vendor[lay] = {"ms": round(ms, 4), "TOP/s": round(2.0 * m * n * k / (ms * 1e-3) / 1e12, 1)}
+        log(f"  vendor int8 gemm {m}^3 {lay}: {ms:.4f} ms  {vendor[lay]['TOP/s']:.0f} TOP/s")
+    out[f"vendor_int8_gemm_{m}"] = vendor
+    del ia, ib, ibn
+    torch.cuda.empty_cache()
     return out
 
 
