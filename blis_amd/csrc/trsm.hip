@@ -34,6 +34,9 @@ constexpr int WPB = 4;          // waves per workgroup
 // Right-hand sides from which the inverted-diagonal path pays: inverting a
 // block costs about as much as solving NB right-hand sides by substitution.
 constexpr int64_t TRSM_INV_MIN_N = 512;
+// Columns per wave when inverting the diagonal blocks: each wave reads the
+// triangle's rows once for all of its columns.
+constexpr int INV_CPW = 2;
 
 __device__ inline double lane_bcast(double v, int l) {
     const long long b = __double_as_longlong(v);
@@ -57,7 +60,12 @@ __device__ inline float lane_bcast(float v, int l) {
 // subtracts its L element times it.
 // Several independent leaves in one launch: leaf y = blockIdx.y starts at
 // a + y*a_step, b + y*b_step, and the last one has nrows_last rows.
-template <typename T, bool LOWER, int CPW>
+// INV: the right-hand side is alpha times the identity (b is written only),
+// so the leaf computes alpha inv(L). A wave's CPW columns lie in one 64-row
+// block p0, and the solution is zero in the blocks on the far side of p0
+// (above it for a lower triangle, below it for an upper one): those are
+// neither solved nor used in updates.
+template <typename T, bool LOWER, int CPW, bool INV = false>
 __global__ __launch_bounds__(64 * WPB) void trsm_leaf_kernel(const T* __restrict__ a, int64_t ars,
                                                              int64_t acs, T* __restrict__ b,
                                                              int64_t brs, int64_t bcs,
@@ -79,13 +87,23 @@ __global__ __launch_bounds__(64 * WPB) void trsm_leaf_kernel(const T* __restrict
         for (int c = 0; c < CPW; ++c) {
             const int r = p * BR + lane;
             const int64_t col = col0 + c;
-            xs[wave][p][c][lane] = (r < nrows && col < n) ? alpha * b[r * brs + col * bcs] : T(0);
+            T v = T(0);
+            if (INV) v = (r < nrows && col < n && r == col) ? alpha : T(0);
+            else if (r < nrows && col < n) v = alpha * b[r * brs + col * bcs];
+            xs[wave][p][c][lane] = v;
         }
     __builtin_amdgcn_wave_barrier();
 
+    // First block in solve order that can be nonzero.
+    int t0 = 0;
+    if (INV) {
+        const int p0 = (int)(col0 / BR);
+        t0 = LOWER ? p0 : nblk - 1 - p0;
+        if (t0 < 0) t0 = 0;
+    }
     T Lr[BR];
 #pragma unroll 1
-    for (int t = 0; t < nblk; ++t) {
+    for (int t = t0; t < nblk; ++t) {
         const int pb = LOWER ? t : nblk - 1 - t;   // block being solved
         const int row = pb * BR + lane;
         T x[CPW];
@@ -93,7 +111,7 @@ __global__ __launch_bounds__(64 * WPB) void trsm_leaf_kernel(const T* __restrict
         for (int c = 0; c < CPW; ++c) x[c] = xs[wave][pb][c][lane];
         // Updates from the blocks solved earlier.
 #pragma unroll 1
-        for (int u = 0; u < t; ++u) {
+        for (int u = t0; u < t; ++u) {
             const int q = LOWER ? u : nblk - 1 - u;
 #pragma unroll
             for (int i = 0; i < BR; ++i) {
@@ -193,16 +211,6 @@ __global__ __launch_bounds__(NB) void trsm_diag_check_kernel(const T* __restrict
     }
 }
 
-// nblk identity blocks of NB x NB, column-major with leading dimension NB.
-template <typename T>
-__global__ void trsm_identity_kernel(T* __restrict__ w, int64_t total) {
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = e % NB, j = (e / NB) % NB;
-        w[e] = (i == j) ? T(1) : T(0);
-    }
-}
-
 // dst(i, j) = src(i, j) for i < rows, j < cols, both general-strided, through
 // 64 x 64 LDS tiles: reads run along i and writes along j, so a copy between
 // column-major and row-major storage is coalesced on both sides.
@@ -284,20 +292,21 @@ inline void* offset(const View& v, int64_t i, int64_t j, size_t es) {
 // Launch `count` leaves of the same shape: leaf y solves the nb-row (the
 // last one nb_last-row) triangle at a + y*a_step for the n columns at
 // b + y*b_step with strides (brs, bcs).
-template <typename T, int CPW>
+template <typename T, int CPW, bool INV = false>
 void leaf_launch(bool lower, bool unit, const void* a, int64_t ars, int64_t acs, void* b,
                  int64_t brs, int64_t bcs, int64_t nb, int64_t n, double alpha, int count,
                  int64_t a_step, int64_t b_step, int64_t nb_last, hipStream_t st) {
+    static_assert(BR % CPW == 0, "a wave's columns must lie in one 64-row block");
     const int64_t waves = (n + CPW - 1) / CPW;
     const dim3 grid((unsigned)((waves + WPB - 1) / WPB), (unsigned)count), block(64 * WPB);
     if (lower)
-        hipLaunchKernelGGL((trsm_leaf_kernel<T, true, CPW>), grid, block, 0, st, (const T*)a, ars,
-                           acs, (T*)b, brs, bcs, (int)nb, n, (T)alpha, (int)unit, a_step, b_step,
-                           (int)nb_last);
+        hipLaunchKernelGGL((trsm_leaf_kernel<T, true, CPW, INV>), grid, block, 0, st,
+                           (const T*)a, ars, acs, (T*)b, brs, bcs, (int)nb, n, (T)alpha,
+                           (int)unit, a_step, b_step, (int)nb_last);
     else
-        hipLaunchKernelGGL((trsm_leaf_kernel<T, false, CPW>), grid, block, 0, st, (const T*)a,
-                           ars, acs, (T*)b, brs, bcs, (int)nb, n, (T)alpha, (int)unit, a_step,
-                           b_step, (int)nb_last);
+        hipLaunchKernelGGL((trsm_leaf_kernel<T, false, CPW, INV>), grid, block, 0, st,
+                           (const T*)a, ars, acs, (T*)b, brs, bcs, (int)nb, n, (T)alpha,
+                           (int)unit, a_step, b_step, (int)nb_last);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -507,7 +516,6 @@ void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag, int64
         int* d_ok = (int*)workspace(sizeof(int) * nblk, 15);
         const int64_t total = (int64_t)nblk * NB * NB;
         void* linv = workspace((size_t)total * es, 13);
-        const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 8192);
         const int64_t a_step = NB * (s.a.rs + s.a.cs);
         if (dt == Dt::F64) {
             if (s.lower)
@@ -540,19 +548,16 @@ void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag, int64
         hipEvent_t checked;
         HIP_CHECK(hipEventCreateWithFlags(&checked, hipEventDisableTiming));
         HIP_CHECK(hipEventRecord(checked, st));
-        if (dt == Dt::F64) {
-            hipLaunchKernelGGL(trsm_identity_kernel<double>, dim3(g), dim3(256), 0, st,
-                               (double*)linv, total);
-            HIP_CHECK(hipGetLastError());
-            leaf_launch<double, 2>(s.lower, s.unit, s.a.ptr, s.a.rs, s.a.cs, linv, 1, NB, NB, NB,
-                                   1.0, nblk, a_step, (int64_t)NB * NB, last, st);
-        } else {
-            hipLaunchKernelGGL(trsm_identity_kernel<float>, dim3(g), dim3(256), 0, st,
-                               (float*)linv, total);
-            HIP_CHECK(hipGetLastError());
-            leaf_launch<float, 2>(s.lower, s.unit, s.a.ptr, s.a.rs, s.a.cs, linv, 1, NB, NB, NB,
-                                  1.0, nblk, a_step, (int64_t)NB * NB, last, st);
-        }
+        // The inverses: leaves whose right-hand side is the identity, built
+        // in the kernel, skipping each column's zero blocks.
+        if (dt == Dt::F64)
+            leaf_launch<double, INV_CPW, true>(s.lower, s.unit, s.a.ptr, s.a.rs, s.a.cs, linv, 1,
+                                               NB, NB, NB, 1.0, nblk, a_step, (int64_t)NB * NB,
+                                               last, st);
+        else
+            leaf_launch<float, INV_CPW, true>(s.lower, s.unit, s.a.ptr, s.a.rs, s.a.cs, linv, 1,
+                                              NB, NB, NB, 1.0, nblk, a_step, (int64_t)NB * NB,
+                                              last, st);
         HIP_CHECK(hipEventSynchronize(checked));
         HIP_CHECK(hipEventDestroy(checked));
         if (flags != ok.data()) std::copy(flags, flags + nblk, ok.begin());
