@@ -479,6 +479,14 @@ void compose_pairs(Solve& s, const void* linv, int64_t rows) {
 } // namespace detail_trsm
 using namespace detail_trsm;
 
+// The stream of the diagonal-block check: created once, default priority,
+// not ordered with the legacy default stream.
+hipStream_t check_stream() {
+    static hipStream_t cs = nullptr;
+    if (!cs) HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    return cs;
+}
+
 void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag, int64_t m,
               int64_t n, double alpha, View a, View b, hipStream_t st) {
     if (m <= 0 || n <= 0) return;
@@ -517,37 +525,41 @@ void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag, int64
         const int64_t total = (int64_t)nblk * NB * NB;
         void* linv = workspace((size_t)total * es, 13);
         const int64_t a_step = NB * (s.a.rs + s.a.cs);
+        // The check runs on a side stream, beside the inversion on the main
+        // stream; only the host reads its flags (pinned memory, one wait).
+        hipStream_t cs = check_stream();
+        hipEvent_t ready, checked;
+        HIP_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&checked, hipEventDisableTiming));
+        HIP_CHECK(hipEventRecord(ready, st));
+        HIP_CHECK(hipStreamWaitEvent(cs, ready, 0));
         if (dt == Dt::F64) {
             if (s.lower)
                 hipLaunchKernelGGL((trsm_diag_check_kernel<double, true>), dim3(nblk), dim3(NB), 0,
-                                   st, (const double*)s.a.ptr, s.a.rs, s.a.cs, rows, (int)s.unit,
+                                   cs, (const double*)s.a.ptr, s.a.rs, s.a.cs, rows, (int)s.unit,
                                    d_ok);
             else
                 hipLaunchKernelGGL((trsm_diag_check_kernel<double, false>), dim3(nblk), dim3(NB),
-                                   0, st, (const double*)s.a.ptr, s.a.rs, s.a.cs, rows,
+                                   0, cs, (const double*)s.a.ptr, s.a.rs, s.a.cs, rows,
                                    (int)s.unit, d_ok);
         } else {
             if (s.lower)
                 hipLaunchKernelGGL((trsm_diag_check_kernel<float, true>), dim3(nblk), dim3(NB), 0,
-                                   st, (const float*)s.a.ptr, s.a.rs, s.a.cs, rows, (int)s.unit,
+                                   cs, (const float*)s.a.ptr, s.a.rs, s.a.cs, rows, (int)s.unit,
                                    d_ok);
             else
                 hipLaunchKernelGGL((trsm_diag_check_kernel<float, false>), dim3(nblk), dim3(NB), 0,
-                                   st, (const float*)s.a.ptr, s.a.rs, s.a.cs, rows, (int)s.unit,
+                                   cs, (const float*)s.a.ptr, s.a.rs, s.a.cs, rows, (int)s.unit,
                                    d_ok);
         }
         HIP_CHECK(hipGetLastError());
-        // The check is read back (one wait) while the GPU inverts the blocks;
-        // the flags land in pinned host memory so the copy is asynchronous.
         ok.assign(nblk, 0);
         static int* pinned = nullptr;
         constexpr int PINNED_FLAGS = 1 << 16;
         if (!pinned) HIP_CHECK(hipHostMalloc((void**)&pinned, sizeof(int) * PINNED_FLAGS));
         int* flags = nblk <= PINNED_FLAGS ? pinned : ok.data();
-        HIP_CHECK(hipMemcpyAsync(flags, d_ok, sizeof(int) * nblk, hipMemcpyDeviceToHost, st));
-        hipEvent_t checked;
-        HIP_CHECK(hipEventCreateWithFlags(&checked, hipEventDisableTiming));
-        HIP_CHECK(hipEventRecord(checked, st));
+        HIP_CHECK(hipMemcpyAsync(flags, d_ok, sizeof(int) * nblk, hipMemcpyDeviceToHost, cs));
+        HIP_CHECK(hipEventRecord(checked, cs));
         // The inverses: leaves whose right-hand side is the identity, built
         // in the kernel, skipping each column's zero blocks.
         if (dt == Dt::F64)
@@ -560,6 +572,7 @@ void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag, int64
                                               last, st);
         HIP_CHECK(hipEventSynchronize(checked));
         HIP_CHECK(hipEventDestroy(checked));
+        HIP_CHECK(hipEventDestroy(ready));
         if (flags != ok.data()) std::copy(flags, flags + nblk, ok.begin());
         s.linv = linv;
         s.inv_ok = ok.data();
