@@ -71,7 +71,8 @@ __global__ __launch_bounds__(64 * WPB) void trsm_leaf_kernel(const T* __restrict
                                                              int64_t brs, int64_t bcs,
                                                              int nrows, int64_t n, T alpha,
                                                              int unit_diag, int64_t a_step,
-                                                             int64_t b_step, int nrows_last) {
+                                                             int64_t b_step, int nrows_last,
+                                                             int zgroup) {
     __shared__ T xs[WPB][P][CPW][BR];
     a += (int64_t)blockIdx.y * a_step;
     b += (int64_t)blockIdx.y * b_step;
@@ -159,6 +160,20 @@ __global__ __launch_bounds__(64 * WPB) void trsm_leaf_kernel(const T* __restrict
             const int64_t col = col0 + c;
             if (r < nrows && col < n) b[r * brs + col * bcs] = xs[wave][p][c][lane];
         }
+    // Inverses of nrows-row leaves inside a block of zgroup leaves (INV
+    // only): zero the leaf's columns of that block on the far side of the
+    // diagonal, rows above the leaf (lower) or below it (upper).
+    if (INV && zgroup > 0) {
+        const int d = (int)(blockIdx.y % zgroup);
+        const int z0 = LOWER ? -d * nrows : nrows;
+        const int zn = LOWER ? d * nrows : (zgroup - 1 - d) * nrows;
+        for (int r = lane; r < zn; r += 64)
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) {
+                const int64_t col = col0 + c;
+                if (col < n) b[(int64_t)(z0 + r) * brs + col * bcs] = T(0);
+            }
+    }
 }
 
 // Which diagonal blocks may be solved by multiplying with their inverse.
@@ -262,7 +277,23 @@ struct Solve {
     // 512-row diagonal blocks (512 x 512 each, column-major, ld 512).
     const void* linv2 = nullptr;
     int64_t npair = 0;
+    // Band store of the inverses (when set, linv and linv2 only mark that
+    // the 256- and 512-row inverses exist): element (i, j) of the block
+    // diagonal inverse at band[(i - j + BAND_KL) + j * BAND_LD], so every
+    // diagonal block, and every block within BAND_KL of the diagonal, is a
+    // column-major matrix with leading dimension BAND_LD - 1, and the
+    // blocks along the diagonal are a constant stride apart.
+    const void* band = nullptr;
 };
+
+constexpr int64_t BAND_KL = 2 * NB;
+constexpr int64_t BAND_LD = 2 * BAND_KL + 1;
+
+// Block (r, c) of the inverse in the band store (|r - c| + block size - 1 <= BAND_KL).
+inline View band_view(const Solve& s, int64_t r, int64_t c) {
+    return { (char*)const_cast<void*>(s.band) + ((r - c + BAND_KL) + c * BAND_LD) * (int64_t)dt_size(s.dt),
+             1, BAND_LD - 1 };
+}
 
 // Diagonal quarters of the 512-row inverses from the 256-row ones (block
 // 2p at the top left, 2p+1 at the bottom right of pair p) and zeros in the
@@ -295,18 +326,19 @@ inline void* offset(const View& v, int64_t i, int64_t j, size_t es) {
 template <typename T, int CPW, bool INV = false>
 void leaf_launch(bool lower, bool unit, const void* a, int64_t ars, int64_t acs, void* b,
                  int64_t brs, int64_t bcs, int64_t nb, int64_t n, double alpha, int count,
-                 int64_t a_step, int64_t b_step, int64_t nb_last, hipStream_t st) {
+                 int64_t a_step, int64_t b_step, int64_t nb_last, hipStream_t st,
+                 int zgroup = 0) {
     static_assert(BR % CPW == 0, "a wave's columns must lie in one 64-row block");
     const int64_t waves = (n + CPW - 1) / CPW;
     const dim3 grid((unsigned)((waves + WPB - 1) / WPB), (unsigned)count), block(64 * WPB);
     if (lower)
         hipLaunchKernelGGL((trsm_leaf_kernel<T, true, CPW, INV>), grid, block, 0, st,
                            (const T*)a, ars, acs, (T*)b, brs, bcs, (int)nb, n, (T)alpha,
-                           (int)unit, a_step, b_step, (int)nb_last);
+                           (int)unit, a_step, b_step, (int)nb_last, zgroup);
     else
         hipLaunchKernelGGL((trsm_leaf_kernel<T, false, CPW, INV>), grid, block, 0, st,
                            (const T*)a, ars, acs, (T*)b, brs, bcs, (int)nb, n, (T)alpha,
-                           (int)unit, a_step, b_step, (int)nb_last);
+                           (int)unit, a_step, b_step, (int)nb_last, zgroup);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -325,7 +357,7 @@ void leaf_by_product(const Solve& s, int64_t r0, int64_t nb, double alpha) {
     GemmDesc d;
     d.dt = s.dt;
     d.m = nb; d.n = s.n; d.k = nb;
-    d.a = { (char*)s.linv + (size_t)q * NB * NB * es, 1, NB };
+    d.a = s.band ? band_view(s, r0, r0) : View{ (char*)s.linv + (size_t)q * NB * NB * es, 1, NB };
     d.b = { offset(s.rhs, r0, 0, es), s.rhs.rs, s.rhs.cs };
     d.alpha = alpha;
     d.beta = 0.0;
@@ -396,7 +428,8 @@ void leaf2_by_product(const Solve& s, int64_t r0, double alpha) {
     GemmDesc d;
     d.dt = s.dt;
     d.m = 2 * NB; d.n = s.n; d.k = 2 * NB;
-    d.a = { (char*)s.linv2 + (size_t)(r0 / (2 * NB)) * 4 * NB * NB * es, 1, 2 * NB };
+    d.a = s.band ? band_view(s, r0, r0)
+                 : View{ (char*)s.linv2 + (size_t)(r0 / (2 * NB)) * 4 * NB * NB * es, 1, 2 * NB };
     d.b = { offset(s.rhs, r0, 0, es), s.rhs.rs, s.rhs.cs };
     d.c = { offset(s.x, r0, 0, es), s.x.rs, s.x.cs };
     d.alpha = alpha;
@@ -476,6 +509,37 @@ void compose_pairs(Solve& s, const void* linv, int64_t rows) {
     s.npair = npair;
 }
 
+// The same composition in the band store, for every aligned pair of b-row
+// inverses: 2b-row inverses from b-row ones (b = 64, 128, 256). Both
+// products are one strided launch over the pairs: in the band the pairs are
+// 2b * BAND_LD elements apart, in the triangle 2b (rs + cs).
+void compose_band(const Solve& s, int64_t rows, int64_t b) {
+    const size_t es = dt_size(s.dt);
+    const int64_t npair = rows / (2 * b);
+    if (npair < 1) return;
+    void* t = workspace((size_t)npair * b * b * es, 19);
+    const bool upper = !s.lower;
+    const int64_t step = 2 * b * BAND_LD;
+    GemmDesc d;
+    d.dt = s.dt;
+    d.m = b; d.n = b; d.k = b;
+    // Lower: T_p = L21_p * inv(L11_p). Upper: T_p = U12_p * inv(U22_p).
+    d.a = upper ? View{ offset(s.a, 0, b, es), s.a.rs, s.a.cs }
+                : View{ offset(s.a, b, 0, es), s.a.rs, s.a.cs };
+    d.b = upper ? band_view(s, b, b) : band_view(s, 0, 0);
+    d.c = { t, 1, b };
+    d.alpha = 1.0;
+    d.beta = 0.0;
+    gemm_strided_run(d, (int)npair, 2 * b * (s.a.rs + s.a.cs), step, b * b, s.st);
+    // Lower: -inv(L22_p) * T_p below the diagonal. Upper: -inv(U11_p) * T_p
+    // to its right.
+    d.a = upper ? band_view(s, 0, 0) : band_view(s, b, b);
+    d.b = { t, 1, b };
+    d.c = upper ? band_view(s, 0, b) : band_view(s, b, 0);
+    d.alpha = -1.0;
+    gemm_strided_run(d, (int)npair, step, b * b, step, s.st);
+}
+
 } // namespace detail_trsm
 using namespace detail_trsm;
 
@@ -522,8 +586,13 @@ void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag, int64
         const int nblk = (int)((rows + NB - 1) / NB);
         const int64_t last = rows - (int64_t)(nblk - 1) * NB;
         int* d_ok = (int*)workspace(sizeof(int) * nblk, 15);
+        // Whole 512-row blocks and a unit-stride triangle: the inverses are
+        // built in the band store from 64-row leaves; otherwise 256-row
+        // leaves are inverted directly (ragged last block).
+        const bool use_band = rows % (2 * NB) == 0 && (s.a.rs == 1 || s.a.cs == 1);
         const int64_t total = (int64_t)nblk * NB * NB;
-        void* linv = workspace((size_t)total * es, 13);
+        void* linv = use_band ? workspace((size_t)(rows + 1) * BAND_LD * es, 13)
+                              : workspace((size_t)total * es, 13);
         const int64_t a_step = NB * (s.a.rs + s.a.cs);
         // The check runs on a side stream, beside the inversion on the main
         // stream; only the host reads its flags (pinned memory, one wait).
@@ -562,14 +631,40 @@ void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag, int64
         HIP_CHECK(hipEventRecord(checked, cs));
         // The inverses: leaves whose right-hand side is the identity, built
         // in the kernel, skipping each column's zero blocks.
-        if (dt == Dt::F64)
+        if (use_band) {
+            // 64-row leaves straight into the band (each also zeroes its
+            // columns of the 512-row block on the far side of the
+            // diagonal), then 128- and 256-row inverses by composition.
+            constexpr int64_t LB = 64;
+            s.band = linv;
+            const View b0 = band_view(s, 0, 0);
+            const int count = (int)(rows / LB);
+            const int zgroup = (int)(2 * NB / LB);
+            if (dt == Dt::F64)
+                leaf_launch<double, INV_CPW, true>(s.lower, s.unit, s.a.ptr, s.a.rs, s.a.cs,
+                                                   b0.ptr, 1, b0.cs, LB, LB, 1.0, count,
+                                                   LB * (s.a.rs + s.a.cs), LB * BAND_LD, LB, st,
+                                                   zgroup);
+            else
+                leaf_launch<float, INV_CPW, true>(s.lower, s.unit, s.a.ptr, s.a.rs, s.a.cs,
+                                                  b0.ptr, 1, b0.cs, LB, LB, 1.0, count,
+                                                  LB * (s.a.rs + s.a.cs), LB * BAND_LD, LB, st,
+                                                  zgroup);
+            compose_band(s, rows, LB);
+            compose_band(s, rows, 2 * LB);
+            // The 512-row inverses are used only when every block passes;
+            // they are composed before the wait for the check anyway, so
+            // that the GPU stays busy while the host reads the flags.
+            compose_band(s, rows, NB);
+        } else if (dt == Dt::F64) {
             leaf_launch<double, INV_CPW, true>(s.lower, s.unit, s.a.ptr, s.a.rs, s.a.cs, linv, 1,
                                                NB, NB, NB, 1.0, nblk, a_step, (int64_t)NB * NB,
                                                last, st);
-        else
+        } else {
             leaf_launch<float, INV_CPW, true>(s.lower, s.unit, s.a.ptr, s.a.rs, s.a.cs, linv, 1,
                                               NB, NB, NB, 1.0, nblk, a_step, (int64_t)NB * NB,
                                               last, st);
+        }
         HIP_CHECK(hipEventSynchronize(checked));
         HIP_CHECK(hipEventDestroy(checked));
         HIP_CHECK(hipEventDestroy(ready));
@@ -609,7 +704,12 @@ void trsm_run(Dt dt, side_t side, uplo_t uplo, trans_t trans, diag_t diag, int64
             // in place, which needs a unit stride; a general-stride triangle
             // keeps 256-row leaves with updates between them (the update
             // GEMMs pack it).
-            if (rows >= 2 * NB && (s.a.rs == 1 || s.a.cs == 1)) compose_pairs(s, linv, rows);
+            if (use_band) {
+                s.linv2 = linv;
+                s.npair = rows / (2 * NB);
+            } else if (rows >= 2 * NB && (s.a.rs == 1 || s.a.cs == 1)) {
+                compose_pairs(s, linv, rows);
+            }
             solve(s, 0, rows, alpha);
             copy(s.x, s.b);
             return;
