@@ -593,9 +593,9 @@ SplitResult gemm_split_run(const GemmDesc& d, int nslices, hipStream_t st) {
     //    point take the native path (which propagates Inf/NaN as BLIS does).
     bool finite = true;
     for (int64_t i = 0; i < rows; ++i) finite &= (h_emax[i] != EXP_NONFINITE);
-    //    The automatic choice for FP64 takes the modular scheme when its
-    //    moduli are fewer than the slice scheme's products (crt.hip); a
-    //    negative count forces that scheme with -nslices moduli.
+    //    The automatic choice takes the modular scheme when its moduli are
+    //    fewer than the slice scheme's products (crt.hip); a negative count
+    //    forces that scheme with -nslices moduli.
     int S = nslices, crt_n = 0, crt_sh = 0;
     if (nslices < 0) {
         S = 0;
@@ -607,14 +607,12 @@ SplitResult gemm_split_run(const GemmDesc& d, int nslices, hipStream_t st) {
                                              h_cnt.data() + d.m, d.n));
         S = split_choose_slices(d.dt, gap);
         if (S > split_max_slices(d.dt)) S = 0;
-        if (d.dt == Dt::F64) {
-            int sh = 0;
-            const int n = crt_choose_moduli(d.dt, gap, d.k, &sh);
-            if (n > 0 && (S == 0 || n < S * (S + 1) / 2)) {
-                crt_n = n;
-                crt_sh = sh;
-                S = 0;
-            }
+        int sh = 0;
+        const int n = crt_choose_moduli(d.dt, gap, d.k, &sh);
+        if (n > 0 && (S == 0 || n < S * (S + 1) / 2)) {
+            crt_n = n;
+            crt_sh = sh;
+            S = 0;
         }
     }
     if (finite && crt_n > 0) return crt_run(d, emax, h_emax, crt_n, crt_sh, st);

@@ -111,9 +111,9 @@ int   bli_amd_gemm_batch_plan_get_single_dispatch( void* plan );
    Chinese remainder theorem (k <= 2^17).
    nslices > 0 forces the slice scheme with that many slices (at most 12);
    nslices in [-32, -2] forces the modular scheme with -nslices moduli;
-   nslices == 0 chooses from the operands' exponent range: FP64 takes the
-   modular scheme when its moduli are fewer than the slice scheme's
-   products, FP32 the slice scheme. If the operands hold Inf/NaN, or if
+   nslices == 0 chooses from the operands' exponent range: the modular
+   scheme when its moduli are fewer than the slice scheme's products, else
+   the slice scheme. If the operands hold Inf/NaN, or if
    the choice is past the point where the emulation is slower than native
    MFMA, the native bli_?gemm path runs instead.
    Returns the slice count (> 0), minus the number of moduli (< 0), or 0

@@ -289,26 +289,33 @@ def test_modular_all_moduli_bit_identical(blis, dtype, layout):
     np.testing.assert_array_equal(c.device_view(), ref.astype(dtype))
 
 
-@pytest.mark.parametrize("sigma", [0.0, 4.0, 8.0])
-def test_modular_policy_error_not_above_native(blis, oracle, sigma):
-    """The automatic FP64 choice on u * 2^round(N(0, sigma)) data takes the
-    modular scheme with the policy's moduli; its error against the
-    double-double oracle is no larger than native bli_dgemm's and within the
-    reported bound plus FP64 rounding."""
+@pytest.mark.parametrize("dtype,sigma", [(np.float64, 0.0), (np.float64, 4.0),
+                                         (np.float64, 8.0), (np.float32, 0.0),
+                                         (np.float32, 4.0)])
+def test_modular_policy_error_not_above_native(blis, oracle, dtype, sigma):
+    """The automatic choice on u * 2^round(N(0, sigma)) data takes the
+    modular scheme with the policy's moduli (fewer INT8 products than the
+    slices); its error against the double-double oracle is no larger than
+    native bli_?gemm's and within the reported bound plus rounding of the
+    output type."""
     m, n, k = 128, 96, 4096
-    say(f"split modular policy sigma={sigma} {m}x{n}x{k}")
-    a, b, c = operands(np.float64, "n", "n", m, n, k, 78, "col", sigma=sigma)
-    cn = Mat(m, n, seed=2)
+    is_d = np.dtype(dtype) == np.float64
+    say(f"split modular policy {np.dtype(dtype).name} sigma={sigma} {m}x{n}x{k}")
+    a, b, c = operands(dtype, "n", "n", m, n, k, 78, "col", sigma=sigma)
+    cn = Mat(m, n, dtype=dtype, seed=2)
     a.to_device(); b.to_device(); c.to_device(); cn.to_device()
     gap = max(ozaki.mean_gap(eff(a, "n")), ozaki.mean_gap(eff(b, "n").T))
-    n_crt, sh = ozaki.crt_choose(True, gap, k)
-    got, bound = split(blis, np.float64, "n", "n", m, n, k, 1.0, 0.0, a, b, c, 0)
-    assert n_crt > 0 and got == -n_crt
-    native(blis, np.float64, "n", "n", m, n, k, 1.0, 0.0, a, b, cn)
+    n_crt, sh = ozaki.crt_choose(is_d, gap, k)
+    s_pol = ozaki.choose_slices(is_d, gap)
+    assert n_crt > 0 and n_crt < s_pol * (s_pol + 1) // 2
+    got, bound = split(blis, dtype, "n", "n", m, n, k, 1.0, 0.0, a, b, c, 0)
+    assert got == -n_crt
+    native(blis, dtype, "n", "n", m, n, k, 1.0, 0.0, a, b, cn)
     ae, be = eff(a, "n"), eff(b, "n")
     ref = dd_ref(oracle, ae, be)
-    e_split = np.linalg.norm(c.device_view() - ref)
-    e_nat = np.linalg.norm(cn.device_view() - ref)
+    e_split = np.linalg.norm(c.device_view().astype(np.float64) - ref)
+    e_nat = np.linalg.norm(cn.device_view().astype(np.float64) - ref)
     say(f"  moduli={n_crt} sh={sh} split_err={e_split:.3e} native_err={e_nat:.3e} bound={bound:.3e}")
     assert e_split <= e_nat
-    assert e_split <= bound + 8 * 2.0 ** -53 * np.linalg.norm(np.abs(ae) @ np.abs(be))
+    u = 2.0 ** -53 if is_d else 2.0 ** -24
+    assert e_split <= bound + 8 * u * np.linalg.norm(np.abs(ae) @ np.abs(be))
