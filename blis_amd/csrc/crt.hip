@@ -176,9 +176,12 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(const T* __restrict__ 
             } else {
 #pragma unroll
                 for (int q = 0; q < 16; ++q) {
-                    const double r0 = sym_mod(vm[q], p, ip);
-                    // times 2^ex mod p: |r0 pw| < 2^16, one exact step
-                    res[q] = (int)sym_mod(r0 * (double)pw[l][ex[q]], p, ip);
+                    // One step leaves |r0| < 0.52 p; times 2^ex mod p that
+                    // stays below 2^16, so one more step is exact and lands
+                    // in the symmetric range.
+                    const double r0 = fma(-rint(vm[q] * ip), p, vm[q]);
+                    const double x = r0 * (double)pw[l][ex[q]];
+                    res[q] = (int)fma(-rint(x * ip), p, x);
                 }
             }
             uint32_t wd[4];
