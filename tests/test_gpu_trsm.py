@@ -129,14 +129,20 @@ def test_trsm_config2_residual(blis, dtype, uplo):
 @pytest.mark.parametrize("uplo", [LOWER, UPPER])
 @pytest.mark.parametrize("ta", ["n", "t"])
 @pytest.mark.parametrize("diag", [0, 0x100])
-def test_trsm_many_leaves(blis, oracle, dtype, side, uplo, ta, diag):
+@pytest.mark.parametrize("size", ["ragged", "whole"])
+def test_trsm_many_leaves(blis, oracle, dtype, side, uplo, ta, diag, size):
     """Solves with several diagonal blocks and many right-hand sides, where
     diagonal blocks that pass the dominance check are solved by products with
     their inverses: a dominant triangle (every block takes the product path)
     and one whose rows are not dominant (every block falls back to
-    substitution), ragged in the last block. Same bar as the small cases."""
+    substitution). "ragged": 700 rows, the last block partial (256-row
+    inverses); "whole": 1024 rows, whole 512-row blocks (inverses composed
+    from 64-row ones in the band store). Same bar as the small cases."""
     u = unit_roundoff(dtype)
-    m, n = (700, 530) if side == 0 else (530, 700)
+    if size == "ragged":
+        m, n = (700, 530) if side == 0 else (530, 700)
+    else:
+        m, n = (1024, 600) if side == 0 else (600, 1024)
     mn = m if side == 0 else n
     for kind in ("dominant", "not dominant"):
         say(f"trsm many leaves {kind} {np.dtype(dtype).name} side={side} uplo={uplo:#x} {ta} "
