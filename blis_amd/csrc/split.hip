@@ -596,7 +596,8 @@ SplitResult gemm_split_run(const GemmDesc& d, int nslices, hipStream_t st) {
     //    The automatic choice takes the modular scheme when its moduli are
     //    fewer than the slice scheme's products (crt.hip); a negative count
     //    forces that scheme with -nslices moduli.
-    int S = nslices, crt_n = 0, crt_sh = 0;
+    // One slice runs as two, the fewest the slice kernels take.
+    int S = nslices == 1 ? 2 : nslices, crt_n = 0, crt_sh = 0;
     if (nslices < 0) {
         S = 0;
         crt_n = -nslices;

@@ -109,7 +109,8 @@ int   bli_amd_gemm_batch_plan_get_single_dispatch( void* plan );
    and forms their product exactly from its residues modulo up to 32
    pairwise coprime moduli, one INT8 GEMM per modulus, rebuilt in FP64 by the
    Chinese remainder theorem (k <= 2^17).
-   nslices > 0 forces the slice scheme with that many slices (at most 12);
+   nslices > 0 forces the slice scheme with that many slices (at most 12;
+   1 runs as 2, the fewest the slice kernels take);
    nslices in [-32, -2] forces the modular scheme with -nslices moduli;
    nslices == 0 chooses from the operands' exponent range: the modular
    scheme when its moduli are fewer than the slice scheme's products, else

@@ -232,6 +232,16 @@ def test_quick_returns_and_zero_rows(blis):
     np.testing.assert_array_equal(out, ozaki.split_gemm(eff(a, "n"), eff(b, "n"), 8))
 
 
+def test_one_slice_runs_as_two(blis):
+    m, n, k = 40, 36, 150
+    say(f"split one slice {m}x{n}x{k}")
+    a, b, c = operands(np.float64, "n", "n", m, n, k, 17, "col", sigma=1.0)
+    a.to_device(); b.to_device(); c.to_device()
+    got, bound = split(blis, np.float64, "n", "n", m, n, k, 1.0, 0.0, a, b, c, 1)
+    assert got == 2 and bound > 0
+    assert np.array_equal(c.device_view(), ozaki.split_gemm(eff(a, "n"), eff(b, "n"), 2))
+
+
 def test_host_pointers_are_staged(blis):
     say("split host pointers")
     m, n, k = 40, 44, 100
