@@ -578,24 +578,16 @@ int crt_shift_for(int n, int64_t k) {
     return std::max(1, (int)std::floor(bits / 2.0));
 }
 
-SplitResult crt_run(const GemmDesc& d, int* emax, const std::vector<int>& h_emax, int n, int sh,
+SplitResult crt_run(const GemmDesc& d, int* emax, const int* h_emax, int n, int sh,
                     hipStream_t st) {
     SplitResult res{-n, 0.0};
     int parts = 0;
     const CrtConst* cc = device_const(n, st, &parts);
     // A-priori bound ||C - C~||_F <= |alpha| k 2^(1 - sh) (1 + 2^-sh)
     //   * sqrt(sum_i 4^e_i) * sqrt(sum_j 4^f_j).
-    auto sum4 = [&](int64_t off, int64_t cnt) {
-        long double s = 0.0L;
-        for (int64_t i = 0; i < cnt; ++i) {
-            const int e = h_emax[off + i];
-            if (e != INT_MIN) s += ldexpl(1.0L, 2 * e);
-        }
-        return s;
-    };
     res.err_bound = (double)(std::fabs(d.alpha) * (long double)d.k * ldexpl(1.0L, 1 - sh) *
-                             (1.0L + ldexpl(1.0L, -sh)) * sqrtl(sum4(0, d.m)) *
-                             sqrtl(sum4(d.m, d.n)));
+                             (1.0L + ldexpl(1.0L, -sh)) * sqrtl(sum_pow4(h_emax, d.m)) *
+                             sqrtl(sum_pow4(h_emax + d.m, d.n)));
 
     // Residue planes of op(A) (rows i) and op(B) (columns j).
     const int64_t nkc = (d.k + KC - 1) / KC;

@@ -213,7 +213,10 @@ BLIS_AMD_HIDDEN int split_max_slices(Dt dt);
 // op(A), then n columns of op(B); h_emax the host copy).
 BLIS_AMD_HIDDEN int crt_choose_moduli(Dt dt, double gap_bits, int64_t k, int* sh_out);
 BLIS_AMD_HIDDEN int crt_shift_for(int n, int64_t k);
-BLIS_AMD_HIDDEN SplitResult crt_run(const GemmDesc& d, int* emax, const std::vector<int>& h_emax,
+// sum_i 4^e_i over the exponents e_i != INT_MIN, in long double, from a
+// histogram of the exponents (one ldexpl per distinct exponent).
+BLIS_AMD_HIDDEN long double sum_pow4(const int* e, int64_t cnt);
+BLIS_AMD_HIDDEN SplitResult crt_run(const GemmDesc& d, int* emax, const int* h_emax,
                                     int n, int sh, hipStream_t st);
 
 } // namespace blis_amd

@@ -560,6 +560,35 @@ static double mean_gap(const int* emax, const long long* esum, const int* ecnt, 
     return used ? tot / (double)used : 0.0;
 }
 
+// Pinned host buffer for the statistics read-back (grow-only; every user
+// waits for its copies before the next call can replace it).
+static void* stats_host(size_t bytes) {
+    static void* p = nullptr;
+    static size_t cap = 0;
+    if (cap < bytes) {
+        if (p) HIP_CHECK(hipHostFree(p));
+        p = nullptr;
+        HIP_CHECK(hipHostMalloc(&p, bytes));
+        cap = bytes;
+    }
+    return p;
+}
+
+long double sum_pow4(const int* e, int64_t cnt) {
+    constexpr int LO = -1200, HI = 1200;
+    std::vector<int64_t> h(HI - LO + 1, 0);
+    long double s = 0.0L;
+    for (int64_t i = 0; i < cnt; ++i) {
+        const int v = e[i];
+        if (v == INT_MIN) continue;
+        if (v >= LO && v <= HI) ++h[v - LO];
+        else s += ldexpl(1.0L, 2 * v);
+    }
+    for (int v = LO; v <= HI; ++v)
+        if (h[v - LO]) s += (long double)h[v - LO] * ldexpl(1.0L, 2 * v);
+    return s;
+}
+
 SplitResult gemm_split_run(const GemmDesc& d, int nslices, hipStream_t st) {
     SplitResult res{0, 0.0};
     if (d.m <= 0 || d.n <= 0) return res;
@@ -581,11 +610,15 @@ SplitResult gemm_split_run(const GemmDesc& d, int nslices, hipStream_t st) {
     long long* esum = (long long*)workspace(sizeof(long long) * (size_t)rows + 256, 5);
     launch_stats(d.dt, d.a, d.m, d.k, d.a.rs, d.a.cs, emax, esum, ecnt, st);
     launch_stats(d.dt, d.b, d.n, d.k, d.b.cs, d.b.rs, emax + d.m, esum + d.m, ecnt + d.m, st);
-    std::vector<int> h_emax(rows), h_cnt(rows);
-    std::vector<long long> h_sum(rows);
-    HIP_CHECK(hipMemcpyAsync(h_emax.data(), emax, sizeof(int) * rows, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipMemcpyAsync(h_cnt.data(), ecnt, sizeof(int) * rows, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipMemcpyAsync(h_sum.data(), esum, sizeof(long long) * rows, hipMemcpyDeviceToHost,
+    // Read back into pinned memory: the maxima and counts (adjacent on the
+    // device) in one copy, the exponent sums in another, then one wait.
+    char* hb = (char*)stats_host((size_t)rows * (2 * sizeof(int) + sizeof(long long)));
+    int* h_emax = (int*)hb;
+    int* h_cnt = h_emax + rows;
+    long long* h_sum = (long long*)(hb + (size_t)rows * 2 * sizeof(int));
+    HIP_CHECK(hipMemcpyAsync(h_emax, emax, sizeof(int) * 2 * (size_t)rows, hipMemcpyDeviceToHost,
+                             st));
+    HIP_CHECK(hipMemcpyAsync(h_sum, esum, sizeof(long long) * (size_t)rows, hipMemcpyDeviceToHost,
                              st));
     HIP_CHECK(hipStreamSynchronize(st));
 
@@ -603,9 +636,8 @@ SplitResult gemm_split_run(const GemmDesc& d, int nslices, hipStream_t st) {
         crt_n = -nslices;
         crt_sh = crt_shift_for(crt_n, d.k);
     } else if (S == 0) {
-        const double gap = std::max(mean_gap(h_emax.data(), h_sum.data(), h_cnt.data(), d.m),
-                                    mean_gap(h_emax.data() + d.m, h_sum.data() + d.m,
-                                             h_cnt.data() + d.m, d.n));
+        const double gap = std::max(mean_gap(h_emax, h_sum, h_cnt, d.m),
+                                    mean_gap(h_emax + d.m, h_sum + d.m, h_cnt + d.m, d.n));
         S = split_choose_slices(d.dt, gap);
         if (S > split_max_slices(d.dt)) S = 0;
         int sh = 0;
@@ -624,16 +656,9 @@ SplitResult gemm_split_run(const GemmDesc& d, int nslices, hipStream_t st) {
 
     // 3. A-priori bound ||C - C~||_F <= |alpha| k (S+2) 2^-7S
     //    * sqrt(sum_i 4^e_i) * sqrt(sum_j 4^f_j).
-    auto sum4 = [&](int64_t off, int64_t cnt) {
-        long double s = 0.0L;
-        for (int64_t i = 0; i < cnt; ++i) {
-            const int e = h_emax[off + i];
-            if (e != INT_MIN) s += ldexpl(1.0L, 2 * e);
-        }
-        return s;
-    };
     res.err_bound = (double)(std::fabs(d.alpha) * (long double)d.k * (S + 2) *
-                             ldexpl(1.0L, -7 * S) * sqrtl(sum4(0, d.m)) * sqrtl(sum4(d.m, d.n)));
+                             ldexpl(1.0L, -7 * S) * sqrtl(sum_pow4(h_emax, d.m)) *
+                             sqrtl(sum_pow4(h_emax + d.m, d.n)));
     res.slices = S;
 
     // 4. Slices. Y is the operand whose index runs along C's unit stride.
