@@ -42,21 +42,29 @@ __global__ __launch_bounds__(TPB) void axpyv_vec_kernel(int64_t n, T alpha, cons
                                                         T* __restrict__ y) {
     using vt = typename V16<T>::type;
     constexpr int V = V16<T>::N;
+    constexpr int RUN = 4 * TPB;   // vectors per workgroup per pass
     const int64_t nv = n / V;
-    const int64_t stride = (int64_t)gridDim.x * TPB;
     const vt* xv = reinterpret_cast<const vt*>(x);
     vt* yv = reinterpret_cast<vt*>(y);
-    int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
-    // Four independent 16-byte loads in flight per thread.
-    for (; i + 3 * stride < nv; i += 4 * stride) {
-        vt a0 = xv[i], a1 = xv[i + stride], a2 = xv[i + 2 * stride], a3 = xv[i + 3 * stride];
-        vt b0 = yv[i], b1 = yv[i + stride], b2 = yv[i + 2 * stride], b3 = yv[i + 3 * stride];
-        yv[i] = b0 + alpha * a0;
-        yv[i + stride] = b1 + alpha * a1;
-        yv[i + 2 * stride] = b2 + alpha * a2;
-        yv[i + 3 * stride] = b3 + alpha * a3;
+    // Each workgroup takes contiguous runs of RUN vectors (16 KiB of x and
+    // of y), four independent 16-byte loads per operand in flight per
+    // thread. Measured at n = 2^28 (FP64): 5.45 TB/s, against 5.15 TB/s
+    // with each thread's four vectors a grid apart.
+    for (int64_t base = blockIdx.x * (int64_t)RUN; base < nv; base += (int64_t)gridDim.x * RUN) {
+        const int64_t i = base + threadIdx.x;
+        if (base + RUN <= nv) {
+            const vt a0 = xv[i], a1 = xv[i + TPB], a2 = xv[i + 2 * TPB], a3 = xv[i + 3 * TPB];
+            const vt b0 = yv[i], b1 = yv[i + TPB], b2 = yv[i + 2 * TPB], b3 = yv[i + 3 * TPB];
+            yv[i] = b0 + alpha * a0;
+            yv[i + TPB] = b1 + alpha * a1;
+            yv[i + 2 * TPB] = b2 + alpha * a2;
+            yv[i + 3 * TPB] = b3 + alpha * a3;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (i + q * TPB < nv) yv[i + q * TPB] = yv[i + q * TPB] + alpha * xv[i + q * TPB];
+        }
     }
-    for (; i < nv; i += stride) yv[i] = yv[i] + alpha * xv[i];
     // Tail elements.
     const int64_t t = nv * V + blockIdx.x * (int64_t)TPB + threadIdx.x;
     if (blockIdx.x == 0 && t < n) y[t] = y[t] + alpha * x[t];
@@ -215,18 +223,35 @@ __global__ __launch_bounds__(TPB) void gemv_n_kernel(int64_t m, int64_t n, const
     }
 }
 
+// Sum of the column chunks' partial results, 64 rows per workgroup: the
+// four waves each add a quarter of the chunks in order, and the quarters
+// are added in order (a fixed order, so the result is the same on every
+// run). Measured for m = 16384 and 64 chunks: 5.4 us against 7.9 us with
+// one thread per row over all chunks.
 template <typename T>
 __global__ __launch_bounds__(TPB) void gemv_n_reduce_kernel(int64_t m, int nchunks,
                                                             const T* __restrict__ part, T alpha,
                                                             T beta, T* y, int64_t incy) {
-    for (int64_t r = blockIdx.x * (int64_t)TPB + threadIdx.x; r < m;
-         r += (int64_t)gridDim.x * TPB) {
+    static_assert(TPB == 256, "four waves of 64 rows");
+    __shared__ T sq[4][64];
+    const int lr = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int per = (nchunks + 3) / 4;
+    const int c0 = q * per, c1 = min(nchunks, c0 + per);
+    for (int64_t r0 = blockIdx.x * (int64_t)64; r0 < m; r0 += (int64_t)gridDim.x * 64) {
+        const int64_t r = r0 + lr;
         T s = T(0);
-        for (int c = 0; c < nchunks; ++c) s += part[c * m + r];
-        T* py = y + r * incy;
-        T o = alpha * s;
-        if (beta != T(0)) o += beta * *py;
-        *py = o;
+        if (r < m)
+            for (int c = c0; c < c1; ++c) s += part[c * m + r];
+        sq[q][lr] = s;
+        __syncthreads();
+        if (q == 0 && r < m) {
+            const T sum = ((sq[0][lr] + sq[1][lr]) + sq[2][lr]) + sq[3][lr];
+            T* py = y + r * incy;
+            T o = alpha * sum;
+            if (beta != T(0)) o += beta * *py;
+            *py = o;
+        }
+        __syncthreads();   // sq is rewritten by the next row group
     }
 }
 
@@ -296,9 +321,9 @@ void axpyv_t(int64_t n, double alpha, const void* x, int64_t incx, void* y, int6
     constexpr int V = V16<T>::N;
     const bool vec = incx == 1 && incy == 1 && al16(x) && al16(y);
     if (vec) {
-        // One pass of four 16-byte vectors per thread over the whole vector
-        // (up to 65536 workgroups): measured 5.13 TB/s at n = 2^28 against
-        // 4.59 TB/s with the grid capped at 4096 and threads looping.
+        // Up to 65536 workgroups of four 16-byte vectors per thread: fewer
+        // workgroups looping over more runs measured slower (5.33 TB/s with
+        // 16384, 5.14 with 4096, at n = 2^28).
         const int g = (int)std::max<int64_t>(
             1, std::min<int64_t>((n / V + 1 + 4 * TPB - 1) / (4 * TPB), 65536));
         hipLaunchKernelGGL(axpyv_vec_kernel<T>, dim3(g), dim3(TPB), 0, st, n, (T)alpha,
@@ -369,7 +394,7 @@ void gemv_t_(trans_t trans, int64_t m, int64_t n, double alpha, View a, const vo
                                dim3(TPB), 0, st, my, nx, (const T*)a.ptr, lda, (const T*)x,
                                incx, (T)alpha, (T)beta, (T*)y, incy, cpc, part);
         if (gy > 1) {
-            const int g = blocks_for(my, 1);
+            const int g = (int)std::min<int64_t>((my + 63) / 64, 65536);
             hipLaunchKernelGGL(gemv_n_reduce_kernel<T>, dim3(g), dim3(TPB), 0, st, my,
                                (int)gy, (const T*)part, (T)alpha, (T)beta, (T*)y, incy);
         }
