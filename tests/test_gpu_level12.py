@@ -117,7 +117,10 @@ def _gemv_case(blis, oracle, dtype, ta, m, n, layout, alpha, beta, incx=1, incy=
 @pytest.mark.parametrize("ta", ["n", "t"])
 @pytest.mark.parametrize("layout", ["col", "row", "general"])
 def test_gemv(blis, oracle, dtype, ta, layout):
-    for (m, n) in [(1, 1), (37, 53), (300, 200), (1025, 2049)]:
+    # (3000, 700): vector-aligned columns whose last row block holds whole
+    # and partial row vectors (the no-transpose kernel's two vectors per
+    # thread, 1024 FP64 / 2048 FP32 rows per block)
+    for (m, n) in [(1, 1), (37, 53), (300, 200), (1025, 2049), (3000, 700)]:
         _gemv_case(blis, oracle, dtype, ta, m, n, layout, 0.7, 1.3)
     _gemv_case(blis, oracle, dtype, ta, 129, 77, layout, -1.0, 0.0)
     _gemv_case(blis, oracle, dtype, ta, 129, 77, layout, 2.0, 0.5, incx=2, incy=3)
