@@ -315,41 +315,35 @@ int gemm_default_tile(Dt dt, int64_t m, int64_t n, int64_t k) {
 
 // Number of k partitions for a product whose output is small against its
 // depth (m, n <= k/4, k >= 2048): such products have too few output tiles to
-// fill 256 CUs and a long serial k loop in each. Every partition is at least
-// 1024 deep, and there are at most 64. The count depends on the shape only,
-// so every tile gives the same partition.
-//   FP64: about 512 workgroups of the tile gemm_split_tile gives (128x128
-//   when m, n >= 1024, else 64x64). Measured on MI355X (device time):
-//   1024x1024x8192 284 us at 8 partitions of 128x128 tiles, against 420 us
-//   at the earlier rule's 2 partitions of the selector's 32x32 tiles;
-//   256x256x32768 97 against 108 us and 128x128x65536 74 against 82 us
-//   (same partitions, 64x64 tiles of BK 32).
-//   FP32: about 2048 workgroups of 32x32 outputs (the rule FP64 used
-//   before; measured with the default tile: 64x64x262144 7.64 -> 0.28 ms at
-//   32 partitions, 128x128x65536 1.95 -> 0.09 ms, 256x256x32768 0.97 ->
-//   0.11 ms, 512x512x16384 0.49 -> 0.20 ms, 1024x1024x8192 0.42 -> 0.38 ms
-//   at 2; 2048x2048x4096 gains nothing).
-static int gemm_default_split(Dt dt, int64_t m, int64_t n, int64_t k) {
+// fill 256 CUs and a long serial k loop in each. The count aims at about 512
+// workgroups of the tile gemm_split_tile gives (128x128 when m, n >= 1024,
+// else 64x64), keeps every partition at least 1024 deep and uses at most 64.
+// It depends on the shape only, so every tile gives the same partition.
+// Measured on MI355X (device time, default path, against the earlier rule of
+// about 2048 workgroups of 32x32 outputs on the selector's tile):
+//   FP64 1024x1024x8192 293 against 378-420 us (8 partitions of 128x128
+//   tiles against 2), 512x512x16384 185 against 202 us, 128x128x65536 75
+//   against 82 us;
+//   FP32 1024x1024x8192 160 against 189 us, 256x256x32768 64 against 72 us,
+//   128x128x65536 53 against 57 us.
+static int gemm_default_split(int64_t m, int64_t n, int64_t k) {
     if (k < 2048 || 4 * std::max(m, n) > k) return 1;
-    int64_t s;
-    if (dt == Dt::F64) {
-        const int64_t bt = (m >= 1024 && n >= 1024) ? 128 : 64;
-        s = std::max<int64_t>(1, 512 / (((m + bt - 1) / bt) * ((n + bt - 1) / bt)));
-    } else {
-        s = std::max<int64_t>(1, 2048 / (((m + 31) / 32) * ((n + 31) / 32)));
-    }
+    const int64_t bt = (m >= 1024 && n >= 1024) ? 128 : 64;
+    int64_t s = std::max<int64_t>(1, 512 / (((m + bt - 1) / bt) * ((n + bt - 1) / bt)));
     s = std::min<int64_t>(s, k / 1024);
     return (int)std::min<int64_t>(s, 64);
 }
 
-// FP64 tile for a product that gemm_default_split splits: 128x128 tiles of
-// eight 64x32 waves when m, n >= 1024; 64x64 tiles of BK 32 when there are
-// at least four of them; else 32x32 tiles (table tiles 6, 8 and 4). A
-// single 64x64 tile leaves too few workgroups: 64x64x262144 took 222 us on
-// it against 182 us on 32x32 tiles.
-static int gemm_split_tile(int64_t m, int64_t n) {
-    if (m >= 1024 && n >= 1024) return 6;
-    return ((m + 63) / 64) * ((n + 63) / 64) >= 4 ? 8 : 4;
+// Tile for a product that gemm_default_split splits: 128x128 tiles when
+// m, n >= 1024 (FP64 table tile 6, eight 64x32 waves; FP32 tile 0, four
+// 64x64 waves); 64x64 tiles when there are at least four of them (FP64 tile
+// 8, BK 32; FP32 tile 3); else 32x32 tiles (tile 4). A single 64x64 tile
+// leaves too few workgroups: FP64 64x64x262144 took 222 us on it against
+// 182 us on 32x32 tiles.
+static int gemm_split_tile(Dt dt, int64_t m, int64_t n) {
+    if (m >= 1024 && n >= 1024) return dt == Dt::F64 ? 6 : 0;
+    if (((m + 63) / 64) * ((n + 63) / 64) >= 4) return dt == Dt::F64 ? 8 : 3;
+    return 4;
 }
 
 // Process-wide override used when a caller passes no configuration
@@ -439,12 +433,11 @@ void gemm_run(const GemmDesc& d, const GemmConfig* cfg, hipStream_t st) {
         const bool vc = aligned16(pl.c) && pl.c_si == 1 && pl.c_sj % (d.dt == Dt::F64 ? 2 : 4) == 0;
         picked = config_of(selector_pick(d.dt, pl.m, pl.n, pl.k, pl.a_k, pl.b_k, va, vb, vc));
         picked.split_k = g_override.split_k > 1 ? g_override.split_k
-                                                : gemm_default_split(d.dt, pl.m, pl.n, pl.k);
-        // An FP64 product split by the shape rule takes the tile measured
-        // for it, unless the selector timed every configuration itself.
-        if (g_override.split_k <= 1 && picked.split_k > 1 && d.dt == Dt::F64 &&
-            selector_mode() != SELECT_EXHAUSTIVE)
-            picked.tile = gemm_split_tile(pl.m, pl.n);
+                                                : gemm_default_split(pl.m, pl.n, pl.k);
+        // A product split by the shape rule takes the tile measured for it,
+        // unless the selector timed every configuration itself.
+        if (g_override.split_k <= 1 && picked.split_k > 1 && selector_mode() != SELECT_EXHAUSTIVE)
+            picked.tile = gemm_split_tile(d.dt, pl.m, pl.n);
         cfg = &picked;
     }
     Scratch sc;
@@ -604,8 +597,8 @@ int bli_amd_gemm_noalloc(num_t dt, trans_t transa, trans_t transb, dim_t m, dim_
     Plan pl = layout(g);
     GemmConfig cfg;
     cfg.tile = gemm_default_tile(d, pl.m, pl.n, pl.k);
-    cfg.split_k = gemm_default_split(d, pl.m, pl.n, pl.k);
-    if (cfg.split_k > 1 && d == Dt::F64) cfg.tile = gemm_split_tile(pl.m, pl.n);
+    cfg.split_k = gemm_default_split(pl.m, pl.n, pl.k);
+    if (cfg.split_k > 1) cfg.tile = gemm_split_tile(d, pl.m, pl.n);
     // Scratch this call needs, carved as Scratch::take does (256-byte
     // aligned pieces): packed general-stride operands and split-k slices.
     const size_t es = dt_size(d);
